@@ -1,0 +1,229 @@
+#!/usr/bin/env python3
+"""Benchmark: field pts x waves/s and rel-L2 vs the FP64 oracle, direct FP32
+form vs MFMA GEMM form, on one MI355X.
+
+Workload (one step = one evaluation of the whole grid, tables resident in HBM):
+  sCWatter 3D tile sample — 21 x 21 orders (441), 20 sublayers of 1 um between
+  21 boundaries, slab 64 x 64 x 20 um at z in [-10, 10]; 2MF down + 2MF up
+  eigenmodes per sublayer -> 4 MF^2 = 777,924 waves per internal region,
+  15,559,803 waves in all; field on 256 x 256 x 128 points (x, y in [-32, 32],
+  z in [-10, 10]); seeded synthetic eigen-tables.
+
+The GEMM form is timed as K replays of a captured hipGraph of one run; the
+direct form (seconds per run) is timed over --direct-steps runs.  Device
+time comes from hipEvents recorded on the stream the kernels run on.  The
+CPU baseline is the reference-semantics FP32 port (single thread, like
+cpuEvaluator.cpp) on a bounded sample of the same workload; the same CPU
+leg evaluates the FP64 oracle on seeded sample points, which is what the
+rel-L2 figures are measured against.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--direct-steps D]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "field pts×waves/s + rel-L2 vs FP64 oracle: direct FP32 vs MFMA GEMM form"
+UNIT = "field pts×waves/s"
+PEAK = {"hbm": 8000.0, "fp32": 157.3, "bf16": 2500.0, "fp16": 2500.0}  # GB/s, TFLOP/s (MI355X, dense)
+ACCURACY_BAR = 1e-4
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def time_graph(ex, out, stream, steps, warmup):
+    from scwatter_amd import hip
+    ex.graph_capture(out.ptr, stream.handle)
+    for _ in range(warmup):
+        ex.graph_launch(stream.handle)
+    stream.synchronize()
+    hip.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        ex.graph_launch(stream.handle)
+    e1.record(stream)
+    e1.synchronize()
+    hip.synchronize()
+    wall = time.perf_counter() - t0
+    return e0.elapsed_ms(e1) / steps, wall * 1e3 / steps
+
+
+def time_stage(ex, stage, out, stream, reps):
+    from scwatter_amd import hip
+    e0, e1 = hip.Event(), hip.Event()
+    ex.run_stage(stage, out.ptr, stream.handle)
+    e0.record(stream)
+    for _ in range(reps):
+        ex.run_stage(stage, out.ptr, stream.handle)
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_ms(e1) / reps
+
+
+def roofline(flops, nbytes, ms, peak_flops):
+    t = ms * 1e-3
+    t_hbm = nbytes / (PEAK["hbm"] * 1e9)
+    t_mfma = flops / (peak_flops * 1e12)
+    if t_hbm >= t_mfma:
+        a = nbytes / t / 1e9
+        return {"bound": "hbm", "achieved": round(a, 1), "peak": PEAK["hbm"], "unit": "GB/s",
+                "frac": round(a / PEAK["hbm"], 4), "traffic": None}
+    a = flops / t / 1e12
+    return {"bound": "mfma", "achieved": round(a, 2), "peak": peak_flops, "unit": "TFLOP/s",
+            "frac": round(a / peak_flops, 4), "traffic": None}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--direct-steps", type=int, default=1)
+    ap.add_argument("--sample-points", type=int, default=192)
+    ap.add_argument("--cpu-rows", type=int, default=1)
+    ap.add_argument("--dtypes", default="fp32,bf16x3,bf16,fp16")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the evaluation shards by z-slab with no exchange; see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import oracle
+    from scwatter_amd import Exec, Plan, configs, hip
+
+    t0 = time.perf_counter()
+    cw, (xs, ys, zs) = configs.config2()
+    tables = cw.unpack()
+    del cw
+    log(f"[bench] structure: {tables.W.shape[0]} waves, {tables.layers} boundaries ({time.perf_counter()-t0:.1f} s)")
+    t0 = time.perf_counter()
+    plan = Plan(tables, device=0)
+    log(f"[bench] plan: {plan.device_bytes()/1e9:.2f} GB on device ({time.perf_counter()-t0:.1f} s)")
+    stream = hip.Stream()
+    shape = (zs.size, ys.size, xs.size, 3)
+    out = hip.DeviceArray(shape, np.complex64)
+
+    # ---- seeded sample points (oracle checks) ----
+    rng = np.random.default_rng(20261203)
+    pidx = np.stack([rng.integers(0, n, args.sample_points) for n in (zs.size, ys.size, xs.size)], 1)
+    pts = np.stack([xs[pidx[:, 2]], ys[pidx[:, 1]], zs[pidx[:, 0]]], 1)
+
+    def sample(E):
+        return E[pidx[:, 0], pidx[:, 1], pidx[:, 2]]
+
+    # ---- GEMM form, every dtype ----
+    forms = {}
+    dtypes = [d for d in args.dtypes.split(",") if d]
+    for dt in dtypes:
+        t0 = time.perf_counter()
+        ex = Exec(plan, xs, ys, zs, form="gemm", dtype=dt)
+        info = ex.info()
+        ms, wall_ms = time_graph(ex, out, stream, args.steps, args.warmup)
+        E = out.to_host()
+        stages = {}
+        for st, name in ((1, "coef"), (2, "xcontract"), (3, "main")):
+            sms = time_stage(ex, st, out, stream, max(3, args.steps // 2))
+            fl, by = ex.stage_info(st)
+            stages[name] = {"ms": round(sms, 4), "flops": fl, "bytes": by}
+        ex.run(out.ptr, stream.handle)  # leave a complete result in `out`
+        stream.synchronize()
+        forms[f"gemm_{dt}"] = {"ms": ms, "wall_ms": wall_ms, "rate": info["units"] / (ms * 1e-3), "info": info,
+                               "stages": stages, "sample": sample(E)}
+        log(f"[bench] gemm {dt}: {ms:.3f} ms/step (wall {wall_ms:.3f}), "
+            f"{info['units'] / (ms * 1e-3):.3e} pts*waves/s, stages " +
+            ", ".join(f"{k} {v['ms']:.3f} ms" for k, v in stages.items()) + f" ({time.perf_counter()-t0:.1f} s)")
+        ex.destroy()
+        del E
+
+    # ---- direct form ----
+    t0 = time.perf_counter()
+    exw = Exec(plan, xs[:8], ys[:8], zs[:1], form="direct")  # load the kernel
+    exw.run_host()
+    exw.destroy()
+    ex = Exec(plan, xs, ys, zs, form="direct")
+    info = ex.info()
+    e0, e1 = hip.Event(), hip.Event()
+    hip.synchronize()
+    e0.record(stream)
+    for _ in range(args.direct_steps):
+        ex.run(out.ptr, stream.handle)
+    e1.record(stream)
+    e1.synchronize()
+    dms = e0.elapsed_ms(e1) / args.direct_steps
+    E = out.to_host()
+    forms["direct_fp32"] = {"ms": dms, "rate": info["units"] / (dms * 1e-3), "info": info, "sample": sample(E)}
+    log(f"[bench] direct fp32: {dms:.1f} ms/step, {info['units'] / (dms * 1e-3):.3e} pts*waves/s "
+        f"({time.perf_counter()-t0:.1f} s)")
+    ex.destroy()
+    del E
+
+    # ---- CPU leg: FP64 oracle on the sample points (checker) + reference-semantics FP32 baseline ----
+    t0 = time.perf_counter()
+    nthr = min(16, os.cpu_count() or 1)
+    ref = oracle.eval_points_f64(tables.W, tables.z_layers, tables.begin, tables.end, pts, nthreads=nthr)
+    log(f"[bench] oracle at {len(pts)} points ({time.perf_counter()-t0:.1f} s, {nthr} threads)")
+    for k, f in forms.items():
+        f["rel_l2"] = oracle.rel_l2(f["sample"], ref)
+        f["max_abs"] = oracle.max_abs(f["sample"], ref)
+        log(f"[bench] {k}: rel-L2 {f['rel_l2']:.3e} max-abs {f['max_abs']:.3e}")
+    # CPU baseline: cpu_cw_evaluate_xy semantics on `cpu_rows` rows of one internal-region slice.
+    iz = int(np.argmin(np.abs(zs - 0.3)))
+    xs32, ys32, zs32 = xs.astype(np.float32), ys[: args.cpu_rows].astype(np.float32), zs[iz:iz + 1].astype(np.float32)
+    r = oracle.region_of(float(zs32[0]), tables.z_layers)
+    cpu_units = float(xs.size * args.cpu_rows * (tables.end[r] - tables.begin[r]))
+    t0 = time.perf_counter()
+    oracle.refcpu_grid(tables.W, tables.z_layers, tables.begin, tables.end, xs32, ys32, zs32, nthreads=1)
+    cpu_s = time.perf_counter() - t0
+    cpu_rate = cpu_units / cpu_s
+    log(f"[bench] cpu baseline: {cpu_units:.3e} units in {cpu_s:.1f} s -> {cpu_rate:.3e} pts*waves/s")
+
+    # ---- headline: fastest GEMM dtype within the accuracy bar ----
+    ok = [k for k in forms if k.startswith("gemm_") and forms[k]["rel_l2"] <= ACCURACY_BAR]
+    best = max(ok, key=lambda k: forms[k]["rate"]) if ok else max(
+        (k for k in forms if k.startswith("gemm_")), key=lambda k: forms[k]["rate"])
+    B = forms[best]
+    dt = best[len("gemm_"):]
+    dom = max(B["stages"], key=lambda k: B["stages"][k]["ms"])
+    S = B["stages"][dom]
+    peak = PEAK["fp32"] if dt == "fp32" or dom != "main" else PEAK["bf16"]
+    roof = roofline(S["flops"], S["bytes"], S["ms"], peak)
+    roof["kernel"] = {"main": "main_f32/main_16 (MFMA)", "coef": "coef_kernel", "xcontract": "u_kernel"}[dom]
+    D = forms["direct_fp32"]
+    roof_direct = {"bound": "valu", "achieved": round(D["info"]["flops"] / (D["ms"] * 1e-3) / 1e12, 2),
+                   "peak": PEAK["fp32"], "unit": "TFLOP/s",
+                   "frac": round(D["info"]["flops"] / (D["ms"] * 1e-3) / 1e12 / PEAK["fp32"], 4),
+                   "note": "36 algorithmic FP32 flops + 3 transcendentals per (point, wave)"}
+    line = {
+        "metric": METRIC, "value": B["rate"], "unit": UNIT, "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": B["ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": dt, "data": "synthetic",
+        "config": {"workload": "sCWatter tile sample 21x21 orders, 20 sublayers, 15,559,803 waves; "
+                               "grid 256x256x128", "units_per_step": B["info"]["units"], "form": "gemm",
+                   "gemm_dtype": dt},
+        "rel_l2": B["rel_l2"], "max_abs": B["max_abs"], "accuracy_sample_points": len(pts),
+        "gemm_over_direct": B["rate"] / D["rate"],
+        "forms": {k: {"ms_per_step": round(f["ms"], 4), "rate": f["rate"], "rel_l2": f["rel_l2"],
+                      "max_abs": f["max_abs"], **({"stages_ms": {s: v["ms"] for s, v in f["stages"].items()}}
+                                                  if "stages" in f else {})} for k, f in forms.items()},
+        "roofline": roof, "roofline_direct": roof_direct,
+        "cpu_baseline": {"value": cpu_rate, "unit": UNIT, "cores": 1, "kind": "port",
+                         "sample": f"cpu_cw_evaluate_xy semantics (FP32, 1 thread) on {args.cpu_rows} row(s) x "
+                                   f"{xs.size} points of the z={zs[iz]:.3f} slice (region {r}, "
+                                   f"{tables.end[r] - tables.begin[r]} waves) = {cpu_units:.3e} units"},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,144 @@
+/*
+ * scwatter_amd — MI355X (gfx950) plane-wave field evaluator, C ABI.
+ *
+ * Evaluates  E(r) = sum_w E0_w exp(i k_w . r)  over the plane waves of the
+ * layer region that holds r, on rectilinear 3D grids, with hand-written HIP
+ * kernels: a direct per-point summation form and a separable MFMA GEMM form
+ * E_z = X . (A o e^{i kz z}) . Y^T.
+ *
+ * These entry points replace the reference's evaluator module:
+ *   cw_allocate / cw_unpack          cpuEvaluator.cpp:34-102   -> scw_cw_unpack
+ *   gpu_initialize                   gpuEvaluator.cu:15-26     -> scw_plan_create
+ *   gpu_cw_evaluate, kernel_{xy,xz,yz} gpuEvaluator.cu:70-165  -> scw_exec_create + scw_exec_run
+ *   cpu_cw_evaluate_{xy,yz,xz}       cpuEvaluator.cpp:294-407  -> scw_exec_create + scw_exec_run
+ *   CoupledWaveStructure::load/save  CoupledWaveStructure.h:72-134 -> scw_cw_load / scw_cw_save
+ * The reference's own C++ signatures are kept in scw_evaluator.hpp, which is
+ * implemented on top of this ABI.
+ *
+ * Conventions
+ *  - Every function returns SCW_OK (0) or a negative status; it never exits
+ *    or aborts.  scw_last_error() gives a message for the calling thread.
+ *  - Output layout (the reference's AoS, gpuEvaluator.cu:76): complex64
+ *    [nz][ny][nx][3] = per point Ex.re Ex.im Ey.re Ey.im Ez.re Ez.im (float32).
+ *    A reference xy plane is nz = 1, an xz plane ny = 1, a yz plane nx = 1.
+ *  - Region of a point: 1 + the largest l with z >= z_layers[l] - region_tol
+ *    (0 if none), compared in double precision.  region_tol = 0 is the CPU
+ *    rule (cpuEvaluator.cpp:303), region_tol = 0.001 the GPU rule
+ *    (gpuEvaluator.cu:34).
+ *  - Waves of region r are waves[waves_begin[r] .. waves_end[r]) (the
+ *    cw_allocate tables, cpuEvaluator.cpp:34-53).
+ */
+#ifndef SCWATTER_AMD_H
+#define SCWATTER_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SCW_ABI_VERSION 1
+
+/* One plane wave as stored in a .cw file (CoupledWaveStructure.h:7-55):
+ * E0.x E0.y E0.z then k.x k.y k.z, each (re, im), FP64.  96 bytes. */
+typedef struct scw_planewave64 {
+    double E0[3][2];
+    double k[3][2];
+} scw_planewave64;
+
+typedef struct scw_plan scw_plan; /* device-resident wave tables of one structure */
+typedef struct scw_exec scw_exec; /* a plan bound to one grid, form and dtype     */
+typedef struct scw_cw scw_cw;     /* an in-memory .cw coupled-wave structure       */
+
+enum scw_status {
+    SCW_OK = 0,
+    SCW_ERR_ARG = -1,         /* invalid argument or table                     */
+    SCW_ERR_NOMEM = -2,       /* host or device allocation failed              */
+    SCW_ERR_DEVICE = -3,      /* HIP runtime error / no device                 */
+    SCW_ERR_IO = -4,          /* file missing, unreadable or truncated         */
+    SCW_ERR_FORMAT = -5,      /* .cw content invalid                           */
+    SCW_ERR_UNSUPPORTED = -6  /* form/dtype not available for this structure  */
+};
+
+enum scw_form {
+    SCW_FORM_AUTO = 0,   /* GEMM when every region on the grid is separable  */
+    SCW_FORM_DIRECT = 1, /* per-point direct summation, FP32                  */
+    SCW_FORM_GEMM = 2    /* separable MFMA form                               */
+};
+
+enum scw_dtype {         /* operand type of the GEMM form's MFMA stage        */
+    SCW_DTYPE_FP32 = 0,  /* v_mfma_f32_16x16x4_f32                            */
+    SCW_DTYPE_BF16 = 1,  /* v_mfma_f32_16x16x32_bf16                          */
+    SCW_DTYPE_BF16X3 = 2,/* split hi+lo bf16: hi*hi + hi*lo + lo*hi           */
+    SCW_DTYPE_FP16 = 3   /* v_mfma_f32_16x16x32_f16                           */
+};
+
+/* ---- errors / device --------------------------------------------------- */
+const char* scw_last_error(void);
+int scw_abi_version(void);
+int scw_device_count(int* count);
+/* hipMemGetInfo of a device (fills the reference's free/total_gpu_memory). */
+int scw_device_mem_info(int device, size_t* free_bytes, size_t* total_bytes);
+
+/* ---- structure plan (replaces gpu_initialize, gpuEvaluator.cu:15-26) ---- */
+/* waves: n_waves records in cw_unpack order; z_layers: layers boundaries;
+ * waves_begin / waves_end: layers + 1 entries each.  All host pointers; the
+ * plan copies what it needs to `device` and keeps no reference to them. */
+int scw_plan_create(int device, const scw_planewave64* waves, size_t n_waves, const double* z_layers,
+                    int layers, const int32_t* waves_begin, const int32_t* waves_end, scw_plan** plan);
+int scw_plan_destroy(scw_plan* plan);
+int scw_plan_layers(const scw_plan* plan, int* layers);
+/* Per-region structure: wave count, distinct kx / ky / kz counts, terms after
+ * merging identical k, ELL width, and whether the GEMM form applies. */
+int scw_plan_region_info(const scw_plan* plan, int region, int64_t* n_waves, int32_t* mx, int32_t* my,
+                         int32_t* mc, int64_t* n_terms, int32_t* width, int32_t* separable);
+int scw_plan_device_bytes(const scw_plan* plan, size_t* bytes);
+
+/* ---- grid execution (replaces gpu_cw_evaluate / kernel_{xy,xz,yz}) ------ */
+/* Rectilinear grid: xs[nx], ys[ny], zs[nz] (host, double).  Builds the
+ * per-grid tables (FP64 on the host, rounded once) and allocates scratch. */
+int scw_exec_create(scw_plan* plan, const double* xs, size_t nx, const double* ys, size_t ny,
+                    const double* zs, size_t nz, double region_tol, int form, int dtype, scw_exec** exec);
+/* Launch the evaluation into out_device (nz*ny*nx*3 complex64) on `stream`
+ * (a hipStream_t, NULL = default stream).  Asynchronous; allocates nothing. */
+int scw_exec_run(scw_exec* exec, void* out_device, void* stream);
+/* Launch one stage of a run (profiling hook): 0 = all, 1 = coefficient
+ * contraction, 2 = X contraction, 3 = main MFMA GEMM (GEMM form); the direct
+ * form has a single stage.  Later stages read what earlier stages wrote. */
+int scw_exec_run_stage(scw_exec* exec, int stage, void* out_device, void* stream);
+/* Same as scw_exec_run, then copy to host memory out_host and wait. */
+int scw_exec_run_host(scw_exec* exec, void* out_host, void* stream);
+int scw_exec_destroy(scw_exec* exec);
+/* form actually used (after AUTO), field pts x waves units, algorithmic FLOPs
+ * and HBM bytes of one run, and the number of kernel launches per run. */
+int scw_exec_info(const scw_exec* exec, int* form, int* dtype, double* units, double* flops, double* bytes,
+                  int* launches);
+/* Algorithmic FLOPs and HBM bytes of one stage (numbering as in
+ * scw_exec_run_stage; 0 = the whole run).  FLOPs count 8 per complex
+ * multiply-add whatever the dtype; bytes count each table read and each
+ * result written once. */
+int scw_exec_stage_info(const scw_exec* exec, int stage, double* flops, double* bytes);
+/* Capture one run into a hipGraph and replay it with scw_exec_graph_launch. */
+int scw_exec_graph_capture(scw_exec* exec, void* out_device, void* stream);
+int scw_exec_graph_launch(scw_exec* exec, void* stream);
+
+/* ---- .cw files (CoupledWaveStructure.h:7-134) --------------------------- */
+int scw_cw_load(const char* path, scw_cw** cw);
+int scw_cw_save(const scw_cw* cw, const char* path);
+int scw_cw_create(scw_cw** cw);
+int scw_cw_destroy(scw_cw* cw);
+int scw_cw_set_incident(scw_cw* cw, const scw_planewave64* Pi, size_t n);
+int scw_cw_add_layer(scw_cw* cw, double z, const scw_planewave64* Pr, size_t n_r, const scw_planewave64* Pt,
+                     size_t n_t);
+int scw_cw_counts(const scw_cw* cw, size_t* n_incident, size_t* n_layers, size_t* n_waves);
+int scw_cw_layer(const scw_cw* cw, size_t layer, double* z, size_t* n_r, size_t* n_t);
+/* cw_allocate + cw_unpack (cpuEvaluator.cpp:34-102) in FP64: waves[n_waves],
+ * z_layers[n_layers], waves_begin/end[n_layers + 1]. */
+int scw_cw_unpack(const scw_cw* cw, scw_planewave64* waves, double* z_layers, int32_t* waves_begin,
+                  int32_t* waves_end);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SCWATTER_AMD_H */

@@ -1,0 +1,474 @@
+// Grid execution: binds a plan to one rectilinear grid, form and dtype.
+//
+// Replaces the per-call work of gpu_cw_evaluate (gpuEvaluator.cu:131-165):
+// instead of cudaMemGetInfo + cudaMalloc + three launches + three blocking
+// D2H copies + cudaFree on every call, all grid tables and scratch are built
+// here once (FP64 on the host, rounded once), and scw_exec_run only launches
+// kernels on the caller's stream — so a run can be captured in a hipGraph.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "scw_internal.hpp"
+
+namespace scw {
+
+static inline int region_of(double z, const std::vector<double>& zl, double tol) {
+    int r = 0;
+    for (size_t l = 0; l < zl.size(); ++l)
+        if (z >= zl[l] - tol) r = (int)l + 1;
+    return r;
+}
+
+static inline uint16_t bf16_rne(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+static inline float bf16_f32(uint16_t h) {
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+static inline uint16_t f16_rne(float f) {
+    _Float16 h = (_Float16)f;
+    uint16_t r;
+    std::memcpy(&r, &h, 2);
+    return r;
+}
+
+static inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+static int prep_direct(scw_exec* ex, const double* xs, const double* ys, const double* zs) {
+    const scw_plan* P = ex->plan;
+    const double xmin = *std::min_element(xs, xs + ex->nx), xmax = *std::max_element(xs, xs + ex->nx);
+    const double ymin = *std::min_element(ys, ys + ex->ny), ymax = *std::max_element(ys, ys + ex->ny);
+    ex->xc = 0.5 * (xmin + xmax);
+    ex->yc = 0.5 * (ymin + ymax);
+    std::vector<float> dx(ex->nx), dy(ex->ny);
+    for (int64_t i = 0; i < ex->nx; ++i) dx[i] = (float)(xs[i] - ex->xc);
+    for (int64_t i = 0; i < ex->ny; ++i) dy[i] = (float)(ys[i] - ex->yc);
+    std::vector<int32_t> zr(ex->zregion.begin(), ex->zregion.end());
+    std::vector<unsigned char> regs(direct_region_bytes() * (P->layers + 1));
+    int64_t maxw = 0;
+    for (int r = 0; r <= P->layers; ++r) {
+        const RegionPlan& R = P->regions[r];
+        direct_region_fill(&regs[direct_region_bytes() * r], R.wbegin, R.wend, R.real_transverse ? 1 : 0);
+    }
+    for (int64_t z = 0; z < ex->nz; ++z) {
+        const RegionPlan& R = P->regions[ex->zregion[z]];
+        maxw = std::max(maxw, R.wend - R.wbegin);
+    }
+    const int64_t pts = ex->nx * ex->ny;
+    ex->direct_ppt = pts >= 1024 ? 4 : 1;
+    const int64_t bps = (pts + 256 * ex->direct_ppt - 1) / (256 * ex->direct_ppt);
+    const int64_t blocks = bps * ex->nz;
+    int split = 1;
+    while (blocks * split < 2048 && maxw / (2 * split) >= 1024 && split < 64) split *= 2;
+    ex->direct_split = split;
+    int st;
+    if ((st = ex->d_dx.upload(dx.data(), dx.size() * 4)) != SCW_OK) return st;
+    if ((st = ex->d_dy.upload(dy.data(), dy.size() * 4)) != SCW_OK) return st;
+    if ((st = ex->d_z.upload(zs, ex->nz * sizeof(double))) != SCW_OK) return st;
+    if ((st = ex->d_zslab.upload(zr.data(), zr.size() * 4)) != SCW_OK) return st;
+    if ((st = ex->d_regions.upload(regs.data(), regs.size())) != SCW_OK) return st;
+    if (split > 1 && (st = ex->d_C.alloc((size_t)split * pts * ex->nz * 24)) != SCW_OK) return st;
+    ex->flops = 36.0 * ex->units;
+    ex->bytes = 24.0 * (double)pts * ex->nz;
+    ex->stage_flops[0] = ex->stage_flops[1] = ex->flops;
+    ex->stage_bytes[0] = ex->stage_bytes[1] = ex->bytes;
+    ex->launches = split > 1 ? 2 : 1;
+    return SCW_OK;
+}
+
+static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const double* zs) {
+    const scw_plan* P = ex->plan;
+    const int L = P->layers;
+    const int nx = (int)ex->nx, ny = (int)ex->ny, nz = (int)ex->nz;
+    const bool is16 = ex->dtype != SCW_DTYPE_FP32;
+    const int kq = is16 ? 32 : 4;  // MFMA K granule
+    std::vector<char> used(L + 1, 0);
+    for (int z = 0; z < nz; ++z) used[ex->zregion[z]] = 1;
+
+    std::vector<RegionDev> rd(L + 1);
+    std::memset(rd.data(), 0, rd.size() * sizeof(RegionDev));
+    std::vector<float> X;  // float2 pairs
+    std::vector<int64_t> yfrag_off(L + 1, 0);
+    std::vector<float> yf32;
+    std::vector<uint16_t> yhi, ylo;
+    const int ytiles = (ny + 15) / 16;
+    int kmax = 0;
+    ex->my_max = 0;
+    for (int r = 0; r <= L; ++r) {
+        if (!used[r]) continue;
+        const RegionPlan& R = P->regions[r];
+        RegionDev& d = rd[r];
+        d.ell_off = R.ell_off;
+        d.ellc_off = R.ellc_off;
+        d.cells = R.mx * R.my;
+        d.width = R.width;
+        d.mc = R.mc;
+        d.dense = R.dense ? 1 : 0;
+        d.mx = R.mx;
+        d.my = R.my;
+        d.nslots = R.width <= 128 ? 1 : std::min(16, (R.width + 95) / 96);
+        ex->nslots_max = std::max(ex->nslots_max, d.nslots);
+        ex->my_max = std::max(ex->my_max, R.my);
+        // X_r[a][x] = exp(i kx_a x)
+        d.x_off = (int64_t)X.size() / 2;
+        X.resize(X.size() + (size_t)2 * R.mx * nx);
+        for (int a = 0; a < R.mx; ++a)
+            for (int i = 0; i < nx; ++i) {
+                const cd v = std::exp(cd(0, 1) * R.kx[a] * xs[i]);
+                if (!std::isfinite(v.real()) || !std::isfinite(v.imag()) || std::abs(v) > 1e30)
+                    return fail(SCW_ERR_UNSUPPORTED, "region %d: exp(i kx x) outside FP32 range", r);
+                X[2 * (d.x_off + (int64_t)a * nx + i)] = (float)v.real();
+                X[2 * (d.x_off + (int64_t)a * nx + i) + 1] = (float)v.imag();
+            }
+        // Y fragments: Yp[y][2b] = Re exp(i ky_b y), Yp[y][2b+1] = Im.
+        const int kp = R.my > 0 ? round_up(2 * R.my, kq) : 0;
+        kmax = std::max(kmax, kp);
+        std::vector<double> Yp((size_t)ny * std::max(kp, 1), 0.0);
+        for (int j = 0; j < ny; ++j)
+            for (int b = 0; b < R.my; ++b) {
+                const cd v = std::exp(cd(0, 1) * R.ky[b] * ys[j]);
+                if (!std::isfinite(v.real()) || !std::isfinite(v.imag()) || std::abs(v) > 1e30)
+                    return fail(SCW_ERR_UNSUPPORTED, "region %d: exp(i ky y) outside FP32 range", r);
+                Yp[(size_t)j * kp + 2 * b] = v.real();
+                Yp[(size_t)j * kp + 2 * b + 1] = v.imag();
+            }
+        auto yval = [&](int y, int k) { return (y < ny && k < kp) ? Yp[(size_t)y * kp + k] : 0.0; };
+        if (!is16) {
+            yfrag_off[r] = (int64_t)yf32.size();
+            const int K4 = kp / 4;
+            for (int yt = 0; yt < ytiles; ++yt)
+                for (int s = 0; s < K4; ++s)
+                    for (int l = 0; l < 64; ++l) yf32.push_back((float)yval(16 * yt + (l & 15), 4 * s + (l >> 4)));
+        } else {
+            yfrag_off[r] = (int64_t)yhi.size();
+            const int K32 = kp / 32;
+            for (int yt = 0; yt < ytiles; ++yt)
+                for (int s = 0; s < K32; ++s)
+                    for (int l = 0; l < 64; ++l)
+                        for (int e = 0; e < 8; ++e) {
+                            const double v = yval(16 * yt + (l & 15), 32 * s + 8 * (l >> 4) + e);
+                            if (ex->dtype == SCW_DTYPE_FP16) {
+                                yhi.push_back(f16_rne((float)v));
+                                ylo.push_back(0);
+                            } else {
+                                const uint16_t h = bf16_rne((float)v);
+                                yhi.push_back(h);
+                                ylo.push_back(bf16_rne((float)(v - (double)bf16_f32(h))));
+                            }
+                        }
+        }
+    }
+    if (!is16) {
+        ex->kmax = kmax <= 16 ? 16 : kmax <= 32 ? 32 : kmax <= 64 ? 64 : 128;
+    } else {
+        ex->kmax = kmax <= 32 ? 32 : kmax <= 64 ? 64 : kmax <= 96 ? 96 : 128;
+    }
+    if (kmax > 128) return fail(SCW_ERR_UNSUPPORTED, "GEMM form supports at most 64 distinct ky per region");
+
+    // Per-z metadata, Z rows, C and U offsets.
+    std::vector<ZMeta> zm(nz);
+    std::vector<float> Z;
+    int64_t u_off = 0, c_off = 0;
+    double f_coef = 0, f_u = 0, f_main = 0;
+    double b_coef = 0, b_u = 0, b_main = 24.0 * (double)nx * ny * nz;
+    for (int z = 0; z < nz; ++z) {
+        const int r = ex->zregion[z];
+        const RegionPlan& R = P->regions[r];
+        ZMeta& m = zm[z];
+        m.region = r;
+        m.my = R.my;
+        m.mx = R.mx;
+        m.kp = R.my > 0 ? round_up(2 * R.my, kq) : 0;
+        m.u_off = u_off;
+        m.yfrag_off = yfrag_off[r];
+        m.zrow_off = (int64_t)Z.size() / 2;
+        m.c_off = c_off;
+        u_off += (int64_t)R.my * nx * 3;
+        c_off += (int64_t)R.mx * R.my * 3;
+        for (int c = 0; c < R.mc; ++c) {
+            const cd v = std::exp(cd(0, 1) * R.kz[c] * (zs[z] - R.zref));
+            if (!std::isfinite(v.real()) || !std::isfinite(v.imag()) || std::abs(v) > 1e30)
+                return fail(SCW_ERR_UNSUPPORTED, "region %d: exp(i kz (z - zref)) outside FP32 range at z=%g", r,
+                            zs[z]);
+            Z.push_back((float)v.real());
+            Z.push_back((float)v.imag());
+        }
+        f_coef += 24.0 * (double)R.n_terms;
+        f_u += 24.0 * (double)R.mx * R.my * nx;
+        f_main += 24.0 * (double)R.my * nx * ny;
+        const double cbytes = 24.0 * (double)R.mx * R.my;   // C_z, one slot
+        const double ubytes = 24.0 * (double)R.my * nx;      // U_z
+        b_coef += cbytes * (R.width <= 128 ? 1 : std::min(16, (R.width + 95) / 96));
+        b_u += cbytes + ubytes + 8.0 * R.mx * nx;
+        b_main += ubytes;
+    }
+    ex->c_elems = c_off;
+
+    // Coefficient work items: runs of <= 8 consecutive z in one region.
+    ex->items.clear();
+    for (int z = 0; z < nz;) {
+        const int r = ex->zregion[z];
+        int e = z + 1;
+        while (e < nz && e - z < 8 && ex->zregion[e] == r) ++e;
+        const RegionDev& d = rd[r];
+        if (d.cells > 0 && d.width > 0) {
+            const int per = (d.width + d.nslots - 1) / d.nslots;
+            for (int c0 = 0; c0 < d.cells; c0 += 64)
+                for (int s = 0; s < d.nslots; ++s) {
+                    CoefItem it;
+                    it.region = r;
+                    it.cell0 = c0;
+                    it.j0 = std::min(d.width, s * per);
+                    it.j1 = std::min(d.width, (s + 1) * per);
+                    it.z0 = z;
+                    it.nz = e - z;
+                    it.slot = s;
+                    it.pad = 0;
+                    ex->items.push_back(it);
+                }
+            b_coef += 24.0 * (double)d.cells * d.width + 8.0 * d.mc * (e - z);  // ELL amplitudes once per run, Z rows
+        }
+        z = e;
+    }
+    ex->stage_flops[1] = f_coef;
+    ex->stage_flops[2] = f_u;
+    ex->stage_flops[3] = f_main;
+    ex->stage_bytes[1] = b_coef;
+    ex->stage_bytes[2] = b_u;
+    ex->stage_bytes[3] = b_main;
+    ex->stage_flops[0] = ex->flops = f_coef + f_u + f_main;
+    ex->stage_bytes[0] = ex->bytes = b_coef + b_u + b_main;
+    ex->launches = (ex->items.empty() ? 0 : 1) + (ex->my_max > 0 ? 1 : 0) + 1;
+
+    const int gx = (int)((6LL * nx + 255) / 256);
+    int ysplit = (int)std::min<int64_t>(ytiles, std::max<int64_t>(1, (2048 + (int64_t)gx * nz - 1) / ((int64_t)gx * nz)));
+    ex->ytiles_per_split = (ytiles + ysplit - 1) / ysplit;
+    ex->ysplit = (ytiles + ex->ytiles_per_split - 1) / ex->ytiles_per_split;
+
+    int st;
+    if ((st = ex->d_regions.upload(rd.data(), rd.size() * sizeof(RegionDev))) != SCW_OK) return st;
+    if ((st = ex->d_zmeta.upload(zm.data(), zm.size() * sizeof(ZMeta))) != SCW_OK) return st;
+    if ((st = ex->d_items.upload(ex->items.data(), ex->items.size() * sizeof(CoefItem))) != SCW_OK) return st;
+    if ((st = ex->d_Z.upload(Z.data(), Z.size() * sizeof(float))) != SCW_OK) return st;
+    if ((st = ex->d_X.upload(X.data(), X.size() * sizeof(float))) != SCW_OK) return st;
+    if ((st = ex->d_C.alloc((size_t)ex->nslots_max * std::max<int64_t>(c_off, 1) * 8)) != SCW_OK) return st;
+    if ((st = ex->d_U.alloc((size_t)std::max<int64_t>(u_off, 1) * 8)) != SCW_OK) return st;
+    if (!is16) {
+        if ((st = ex->d_Yfrag.upload(yf32.data(), std::max<size_t>(yf32.size(), 1) * 4)) != SCW_OK) return st;
+        ex->ylo_delta = 0;
+    } else {
+        std::vector<uint16_t> both(yhi);
+        both.insert(both.end(), ylo.begin(), ylo.end());
+        if (both.empty()) both.push_back(0);
+        ex->ylo_delta = (int64_t)yhi.size();
+        if ((st = ex->d_Yfrag.upload(both.data(), both.size() * 2)) != SCW_OK) return st;
+    }
+    return SCW_OK;
+}
+
+static void exec_free(scw_exec* ex) {
+    if (ex->graph_exec) (void)hipGraphExecDestroy(ex->graph_exec);
+    if (ex->graph) (void)hipGraphDestroy(ex->graph);
+    if (ex->graph_stream) (void)hipStreamDestroy(ex->graph_stream);
+    for (DevBuf* b : {&ex->d_dx, &ex->d_dy, &ex->d_z, &ex->d_zslab, &ex->d_items, &ex->d_regions, &ex->d_zmeta,
+                      &ex->d_Z, &ex->d_X, &ex->d_C, &ex->d_U, &ex->d_Yfrag, &ex->d_out})
+        b->release();
+    delete ex;
+}
+
+}  // namespace scw
+
+using namespace scw;
+
+extern "C" {
+
+int scw_exec_create(scw_plan* plan, const double* xs, size_t nx, const double* ys, size_t ny, const double* zs,
+                    size_t nz, double region_tol, int form, int dtype, scw_exec** out) {
+    if (!out) return fail(SCW_ERR_ARG, "exec output is NULL");
+    *out = nullptr;
+    if (!plan) return fail(SCW_ERR_ARG, "plan is NULL");
+    if (!xs || !ys || !zs) return fail(SCW_ERR_ARG, "coordinate array is NULL");
+    if (nx == 0 || ny == 0 || nz == 0) return fail(SCW_ERR_ARG, "empty grid (%zu x %zu x %zu)", nx, ny, nz);
+    if (nx > (1u << 24) || ny > (1u << 24) || nz > 65535)
+        return fail(SCW_ERR_ARG, "grid too large (nx, ny <= 2^24, nz <= 65535)");
+    if ((double)nx * ny * nz * 24.0 > 1.0e12) return fail(SCW_ERR_ARG, "grid output exceeds 1 TB");
+    if (!std::isfinite(region_tol)) return fail(SCW_ERR_ARG, "region_tol is not finite");
+    if (form < SCW_FORM_AUTO || form > SCW_FORM_GEMM) return fail(SCW_ERR_ARG, "unknown form %d", form);
+    if (dtype < SCW_DTYPE_FP32 || dtype > SCW_DTYPE_FP16) return fail(SCW_ERR_ARG, "unknown dtype %d", dtype);
+    for (size_t i = 0; i < nx; ++i)
+        if (!std::isfinite(xs[i])) return fail(SCW_ERR_ARG, "xs[%zu] is not finite", i);
+    for (size_t i = 0; i < ny; ++i)
+        if (!std::isfinite(ys[i])) return fail(SCW_ERR_ARG, "ys[%zu] is not finite", i);
+    for (size_t i = 0; i < nz; ++i)
+        if (!std::isfinite(zs[i])) return fail(SCW_ERR_ARG, "zs[%zu] is not finite", i);
+
+    scw_exec* ex = new (std::nothrow) scw_exec();
+    if (!ex) return fail(SCW_ERR_NOMEM, "host allocation failed");
+    ex->plan = plan;
+    ex->nx = (int64_t)nx;
+    ex->ny = (int64_t)ny;
+    ex->nz = (int64_t)nz;
+    ex->dtype = dtype;
+    ex->zregion.resize(nz);
+    bool all_sep = true;
+    int first_bad = -1;
+    double units = 0.0;
+    for (size_t z = 0; z < nz; ++z) {
+        const int r = region_of(zs[z], plan->zl, region_tol);
+        ex->zregion[z] = r;
+        const RegionPlan& R = plan->regions[r];
+        units += (double)nx * ny * (double)(R.wend - R.wbegin);
+        if (!R.separable && all_sep) {
+            all_sep = false;
+            first_bad = r;
+        }
+    }
+    ex->units = units;
+    if (form == SCW_FORM_AUTO) form = all_sep ? SCW_FORM_GEMM : SCW_FORM_DIRECT;
+    if (form == SCW_FORM_GEMM && !all_sep) {
+        const int st = fail(SCW_ERR_UNSUPPORTED, "GEMM form: region %d is not separable (%s)", first_bad,
+                            plan->regions[first_bad].why_not.c_str());
+        delete ex;
+        return st;
+    }
+    ex->form = form;
+    if (form == SCW_FORM_DIRECT) ex->dtype = SCW_DTYPE_FP32;
+
+    DeviceGuard g(plan->device);
+    if (g.st != SCW_OK) {
+        delete ex;
+        return g.st;
+    }
+    int st;
+    try {
+        st = form == SCW_FORM_DIRECT ? prep_direct(ex, xs, ys, zs) : prep_gemm(ex, xs, ys, zs);
+    } catch (const std::bad_alloc&) {
+        st = fail(SCW_ERR_NOMEM, "host allocation failed while building grid tables");
+    }
+    if (st != SCW_OK) {
+        exec_free(ex);
+        return st;
+    }
+    *out = ex;
+    return SCW_OK;
+}
+
+int scw_exec_run(scw_exec* ex, void* out_device, void* stream) {
+    if (!ex) return fail(SCW_ERR_ARG, "exec is NULL");
+    if (!out_device) return fail(SCW_ERR_ARG, "output pointer is NULL");
+    DeviceGuard g(ex->plan->device);
+    if (g.st != SCW_OK) return g.st;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    return ex->form == SCW_FORM_DIRECT ? launch_direct(ex, static_cast<float*>(out_device), s)
+                                       : launch_gemm(ex, static_cast<float*>(out_device), s);
+}
+
+int scw_exec_run_stage(scw_exec* ex, int stage, void* out_device, void* stream) {
+    if (!ex) return fail(SCW_ERR_ARG, "exec is NULL");
+    if (!out_device) return fail(SCW_ERR_ARG, "output pointer is NULL");
+    if (stage < 0 || stage > 3) return fail(SCW_ERR_ARG, "stage %d out of range", stage);
+    DeviceGuard g(ex->plan->device);
+    if (g.st != SCW_OK) return g.st;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    return ex->form == SCW_FORM_DIRECT ? launch_direct(ex, static_cast<float*>(out_device), s)
+                                       : launch_gemm(ex, static_cast<float*>(out_device), s, stage);
+}
+
+int scw_exec_run_host(scw_exec* ex, void* out_host, void* stream) {
+    if (!ex) return fail(SCW_ERR_ARG, "exec is NULL");
+    if (!out_host) return fail(SCW_ERR_ARG, "output pointer is NULL");
+    DeviceGuard g(ex->plan->device);
+    if (g.st != SCW_OK) return g.st;
+    const size_t bytes = (size_t)ex->nx * ex->ny * ex->nz * 24;
+    if (ex->d_out.bytes < bytes) {
+        const int st = ex->d_out.alloc(bytes);
+        if (st != SCW_OK) return st;
+    }
+    int st = scw_exec_run(ex, ex->d_out.p, stream);
+    if (st != SCW_OK) return st;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    SCW_HIP(hipMemcpyAsync(out_host, ex->d_out.p, bytes, hipMemcpyDeviceToHost, s));
+    SCW_HIP(hipStreamSynchronize(s));
+    return SCW_OK;
+}
+
+int scw_exec_destroy(scw_exec* ex) {
+    if (!ex) return SCW_OK;
+    DeviceGuard g(ex->plan->device);
+    exec_free(ex);
+    return SCW_OK;
+}
+
+int scw_exec_info(const scw_exec* ex, int* form, int* dtype, double* units, double* flops, double* bytes,
+                  int* launches) {
+    if (!ex) return fail(SCW_ERR_ARG, "exec is NULL");
+    if (form) *form = ex->form;
+    if (dtype) *dtype = ex->dtype;
+    if (units) *units = ex->units;
+    if (flops) *flops = ex->flops;
+    if (bytes) *bytes = ex->bytes;
+    if (launches) *launches = ex->launches;
+    return SCW_OK;
+}
+
+int scw_exec_stage_info(const scw_exec* ex, int stage, double* flops, double* bytes) {
+    if (!ex) return fail(SCW_ERR_ARG, "exec is NULL");
+    if (stage < 0 || stage > 3) return fail(SCW_ERR_ARG, "stage %d out of range", stage);
+    if (flops) *flops = ex->stage_flops[stage];
+    if (bytes) *bytes = ex->stage_bytes[stage];
+    return SCW_OK;
+}
+
+int scw_exec_graph_capture(scw_exec* ex, void* out_device, void* stream) {
+    if (!ex) return fail(SCW_ERR_ARG, "exec is NULL");
+    if (!out_device) return fail(SCW_ERR_ARG, "output pointer is NULL");
+    DeviceGuard g(ex->plan->device);
+    if (g.st != SCW_OK) return g.st;
+    if (ex->graph_exec) {
+        (void)hipGraphExecDestroy(ex->graph_exec);
+        ex->graph_exec = nullptr;
+    }
+    if (ex->graph) {
+        (void)hipGraphDestroy(ex->graph);
+        ex->graph = nullptr;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (!s) {  // capture needs a non-legacy stream
+        if (!ex->graph_stream) SCW_HIP(hipStreamCreateWithFlags(&ex->graph_stream, hipStreamNonBlocking));
+        s = ex->graph_stream;
+    }
+    SCW_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int st = ex->form == SCW_FORM_DIRECT ? launch_direct(ex, static_cast<float*>(out_device), s)
+                                         : launch_gemm(ex, static_cast<float*>(out_device), s);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &graph);
+    if (st != SCW_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    ex->graph = graph;
+    SCW_HIP(hipGraphInstantiate(&ex->graph_exec, graph, nullptr, nullptr, 0));
+    return SCW_OK;
+}
+
+int scw_exec_graph_launch(scw_exec* ex, void* stream) {
+    if (!ex || !ex->graph_exec) return fail(SCW_ERR_ARG, "no captured graph");
+    DeviceGuard g(ex->plan->device);
+    if (g.st != SCW_OK) return g.st;
+    SCW_HIP(hipGraphLaunch(ex->graph_exec, static_cast<hipStream_t>(stream)));
+    return SCW_OK;
+}
+
+}  // extern "C"

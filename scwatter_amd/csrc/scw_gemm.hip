@@ -1,0 +1,374 @@
+// Separable GEMM form of the field sum (no counterpart in the reference,
+// which only has the direct form gpuEvaluator.cu:28-68).
+//
+// A region whose waves sit on a lattice kx = kx_a, ky = ky_b (every
+// producer emits such sets: sCWatter.cpp:893-899, sCWatterPlane.cpp:254-262,
+// sCWatterLayer.cpp:402-414) satisfies, per field component,
+//
+//     E(x_i, y_j, z) = sum_b Y[j,b] sum_a X[i,a] C_z[a,b],
+//     C_z[a,b]       = sum_c A[a,b,c] exp(i kz_c (z - zref)),
+//     X[i,a] = exp(i kx_a x_i),  Y[j,b] = exp(i ky_b y_j).
+//
+// Three launches per grid:
+//   coef_kernel : C_z for every z of the grid (ELL contraction over the
+//                 region's terms; wave-level work items, FP32 VALU).
+//   u_kernel    : U_z[b][x][comp] = sum_a C_z[a,b,comp] X[a][x]  (FP32 VALU).
+//   main_*      : E_z[y][x][comp] = sum_b Y[y,b] U_z[b][x][comp] on MFMA.
+//
+// The main GEMM is a real GEMM with the complex product folded into K:
+//   E[y][n] = sum_k Yp[y][k] Bp[k][n],  n = 6x + 2comp + (re|im),
+//   Yp[y][2b] = Re Y[y,b], Yp[y][2b+1] = Im Y[y,b],
+//   Bp[2b][n]   = U re/im as stored,  Bp[2b+1][n] = (i U) re/im,
+// so its output rows are already the evaluator's AoS layout.  The MFMA
+// "row" dimension is n (four consecutive n per lane -> 16-byte stores) and
+// the "column" dimension is y.  Y fragments are pre-laid out on the host in
+// the exact lane order of the MFMA B operand, so each is one coalesced load.
+#include "scw_internal.hpp"
+
+namespace scw {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+
+// ---------------------------------------------------------------------------
+// C_z[a,b,comp] partial sums over a term slice, for up to 8 z of one region.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void coef_kernel(const CoefItem* __restrict__ items, int nitems,
+                                                   const RegionDev* __restrict__ regions,
+                                                   const ZMeta* __restrict__ zmeta, const float2* __restrict__ ellA,
+                                                   const int32_t* __restrict__ ellC, const float2* __restrict__ Z,
+                                                   int64_t c_elems, float2* __restrict__ C) {
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int it = blockIdx.x * 4 + wv;
+    if (it >= nitems) return;
+    const CoefItem I = items[it];
+    const RegionDev R = regions[I.region];
+    const int cell = I.cell0 + lane;
+    const bool active = cell < R.cells;
+    const int cl = active ? cell : 0;
+    int64_t zrow[8];
+#pragma unroll
+    for (int zi = 0; zi < 8; ++zi) zrow[zi] = zmeta[I.z0 + min(zi, I.nz - 1)].zrow_off;
+    float acc[8][6];
+#pragma unroll
+    for (int zi = 0; zi < 8; ++zi)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[zi][c] = 0.0f;
+
+    for (int j = I.j0; j < I.j1; ++j) {
+        const int64_t slot = (int64_t)j * R.cells + cl;
+        const float2* a = ellA + (R.ell_off + slot) * 3;
+        const float2 a0 = a[0], a1 = a[1], a2 = a[2];
+        const int c = R.dense ? j : ellC[R.ellc_off + slot];
+#pragma unroll
+        for (int zi = 0; zi < 8; ++zi) {
+            if (zi < I.nz) {
+                const float2 f = Z[zrow[zi] + c];
+                acc[zi][0] = fmaf(a0.x, f.x, fmaf(-a0.y, f.y, acc[zi][0]));
+                acc[zi][1] = fmaf(a0.x, f.y, fmaf(a0.y, f.x, acc[zi][1]));
+                acc[zi][2] = fmaf(a1.x, f.x, fmaf(-a1.y, f.y, acc[zi][2]));
+                acc[zi][3] = fmaf(a1.x, f.y, fmaf(a1.y, f.x, acc[zi][3]));
+                acc[zi][4] = fmaf(a2.x, f.x, fmaf(-a2.y, f.y, acc[zi][4]));
+                acc[zi][5] = fmaf(a2.x, f.y, fmaf(a2.y, f.x, acc[zi][5]));
+            }
+        }
+    }
+    if (!active) return;
+#pragma unroll
+    for (int zi = 0; zi < 8; ++zi) {
+        if (zi < I.nz) {
+            float2* o = C + (int64_t)I.slot * c_elems + zmeta[I.z0 + zi].c_off + (int64_t)cell * 3;
+            o[0] = make_float2(acc[zi][0], acc[zi][1]);
+            o[1] = make_float2(acc[zi][2], acc[zi][3]);
+            o[2] = make_float2(acc[zi][4], acc[zi][5]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// U_z[b][x][comp] = sum_a (sum_slots C_z[a,b,comp]) X[a][x]
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void u_kernel(const ZMeta* __restrict__ zmeta, const RegionDev* __restrict__ regions,
+                                                const float2* __restrict__ C, int64_t c_elems,
+                                                const float2* __restrict__ X, int nx, float2* __restrict__ U) {
+    const int z = blockIdx.z, b = blockIdx.y;
+    const ZMeta zm = zmeta[z];
+    if (b >= zm.my) return;
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= nx) return;
+    const RegionDev R = regions[zm.region];
+    float acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int a = 0; a < R.mx; ++a) {
+        const float2 xv = X[R.x_off + (int64_t)a * nx + x];
+        const int64_t cb = zm.c_off + ((int64_t)a * R.my + b) * 3;
+        float2 c0 = make_float2(0, 0), c1 = c0, c2 = c0;
+        for (int s = 0; s < R.nslots; ++s) {
+            const float2* cs = C + (int64_t)s * c_elems + cb;
+            const float2 t0 = cs[0], t1 = cs[1], t2 = cs[2];
+            c0.x += t0.x; c0.y += t0.y;
+            c1.x += t1.x; c1.y += t1.y;
+            c2.x += t2.x; c2.y += t2.y;
+        }
+        acc[0] = fmaf(c0.x, xv.x, fmaf(-c0.y, xv.y, acc[0]));
+        acc[1] = fmaf(c0.x, xv.y, fmaf(c0.y, xv.x, acc[1]));
+        acc[2] = fmaf(c1.x, xv.x, fmaf(-c1.y, xv.y, acc[2]));
+        acc[3] = fmaf(c1.x, xv.y, fmaf(c1.y, xv.x, acc[3]));
+        acc[4] = fmaf(c2.x, xv.x, fmaf(-c2.y, xv.y, acc[4]));
+        acc[5] = fmaf(c2.x, xv.y, fmaf(c2.y, xv.x, acc[5]));
+    }
+    float2* o = U + zm.u_off + ((int64_t)b * nx + x) * 3;
+    o[0] = make_float2(acc[0], acc[1]);
+    o[1] = make_float2(acc[2], acc[3]);
+    o[2] = make_float2(acc[4], acc[5]);
+}
+
+// Value of the main GEMM's B operand Bp[k][n] read from U row b = k >> 1.
+__device__ inline float bp_value(const float* __restrict__ Ub, int64_t ncols, int k, int64_t n) {
+    const float2 u = *reinterpret_cast<const float2*>(Ub + (int64_t)(k >> 1) * ncols + (n & ~(int64_t)1));
+    const bool odd = (n & 1) != 0;
+    return (k & 1) == 0 ? (odd ? u.y : u.x) : (odd ? u.x : -u.y);
+}
+
+template <bool VEC>
+__device__ inline void store_tile(float* __restrict__ orow, int64_t n, int64_t ncols, const floatx4 v) {
+    if (VEC) {
+        if (n < ncols) *reinterpret_cast<float4*>(orow + n) = make_float4(v[0], v[1], v[2], v[3]);
+    } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            if (n + r < ncols) orow[n + r] = v[r];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Main GEMM, FP32 operands: v_mfma_f32_16x16x4_f32.
+// Workgroup = 4 waves, each owns 64 output columns (n) of one z-slice and
+// walks y in tiles of 16; its B-operand fragment (from U) stays in registers.
+// ---------------------------------------------------------------------------
+template <int KMAX, bool VEC>
+__global__ __launch_bounds__(256) void main_f32(const ZMeta* __restrict__ zmeta, const float2* __restrict__ U,
+                                                const float* __restrict__ Yfrag, int nx, int ny, int ytiles_per_split,
+                                                float* __restrict__ out) {
+    const int z = blockIdx.y;
+    const ZMeta zm = zmeta[z];
+    const int K4 = zm.kp >> 2;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t ncols = 6LL * nx;
+    const int64_t n0 = (int64_t)blockIdx.x * 256 + wv * 64;
+    if (n0 >= ncols) return;
+    const int li = lane & 15, lk = lane >> 4;
+    const float* Ub = reinterpret_cast<const float*>(U + zm.u_off);
+
+    float uf[4][KMAX / 4];
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+        const int64_t n = n0 + 16 * it + li;
+#pragma unroll
+        for (int s = 0; s < KMAX / 4; ++s) {
+            const int k = 4 * s + lk;
+            uf[it][s] = (s < K4 && (k >> 1) < zm.my && n < ncols) ? bp_value(Ub, ncols, k, n) : 0.0f;
+        }
+    }
+
+    const int ytiles = (ny + 15) >> 4;
+    const int yt0 = blockIdx.z * ytiles_per_split;
+    const int yt1 = min(ytiles, yt0 + ytiles_per_split);
+    const float* Yf = Yfrag + zm.yfrag_off;
+    float* oz = out + (int64_t)z * ny * ncols;
+    for (int yt = yt0; yt < yt1; ++yt) {
+        floatx4 acc[4];
+#pragma unroll
+        for (int it = 0; it < 4; ++it) acc[it] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+        const float* yrow = Yf + (int64_t)yt * K4 * 64 + lane;
+#pragma unroll
+        for (int s = 0; s < KMAX / 4; ++s) {
+            if (s < K4) {
+                const float yv = yrow[s * 64];
+#pragma unroll
+                for (int it = 0; it < 4; ++it)
+                    acc[it] = __builtin_amdgcn_mfma_f32_16x16x4f32(uf[it][s], yv, acc[it], 0, 0, 0);
+            }
+        }
+        const int y = yt * 16 + li;
+        if (y < ny) {
+            float* orow = oz + (int64_t)y * ncols;
+#pragma unroll
+            for (int it = 0; it < 4; ++it) store_tile<VEC>(orow, n0 + 16 * it + 4 * lk, ncols, acc[it]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Main GEMM, 16-bit operands with FP32 accumulation:
+//   MODE 0: bf16      v_mfma_f32_16x16x32_bf16
+//   MODE 1: bf16 x3   hi*hi + hi*lo + lo*hi (hi = rn(v), lo = rn(v - hi))
+//   MODE 2: fp16      v_mfma_f32_16x16x32_f16
+// A-operand lane layout: lane l holds k = 32 s + 8 (l >> 4) + e, e = 0..7.
+// ---------------------------------------------------------------------------
+template <int MODE>
+struct Op16;
+template <>
+struct Op16<0> {
+    typedef bf16x8 vec;
+    typedef __bf16 elem;
+};
+template <>
+struct Op16<1> {
+    typedef bf16x8 vec;
+    typedef __bf16 elem;
+};
+template <>
+struct Op16<2> {
+    typedef halfx8 vec;
+    typedef _Float16 elem;
+};
+
+template <int MODE>
+__device__ inline floatx4 mfma16(const typename Op16<MODE>::vec a, const typename Op16<MODE>::vec b, floatx4 c) {
+    if constexpr (MODE == 2)
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+template <int KMAX, bool VEC, int MODE>
+__global__ __launch_bounds__(256) void main_16(const ZMeta* __restrict__ zmeta, const float2* __restrict__ U,
+                                               const uint16_t* __restrict__ Yfrag, int64_t ylo_delta, int nx,
+                                               int ny, int ytiles_per_split, float* __restrict__ out) {
+    typedef typename Op16<MODE>::vec vec;
+    typedef typename Op16<MODE>::elem elem;
+    constexpr int S = KMAX / 32;
+    const int z = blockIdx.y;
+    const ZMeta zm = zmeta[z];
+    const int K32 = zm.kp >> 5;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t ncols = 6LL * nx;
+    const int64_t n0 = (int64_t)blockIdx.x * 256 + wv * 64;
+    if (n0 >= ncols) return;
+    const int li = lane & 15, lk = lane >> 4;
+    const float* Ub = reinterpret_cast<const float*>(U + zm.u_off);
+
+    vec ahi[4][S], alo[4][S];
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+        const int64_t n = n0 + 16 * it + li;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int k = 32 * s + 8 * lk + e;
+                const float v = (s < K32 && (k >> 1) < zm.my && n < ncols) ? bp_value(Ub, ncols, k, n) : 0.0f;
+                const elem h = (elem)v;
+                ahi[it][s][e] = h;
+                if constexpr (MODE == 1) alo[it][s][e] = (elem)(v - (float)h);
+            }
+        }
+    }
+
+    const int ytiles = (ny + 15) >> 4;
+    const int yt0 = blockIdx.z * ytiles_per_split;
+    const int yt1 = min(ytiles, yt0 + ytiles_per_split);
+    const uint16_t* Yf = Yfrag + zm.yfrag_off;
+    float* oz = out + (int64_t)z * ny * ncols;
+    for (int yt = yt0; yt < yt1; ++yt) {
+        floatx4 acc[4];
+#pragma unroll
+        for (int it = 0; it < 4; ++it) acc[it] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+        const uint16_t* yrow = Yf + ((int64_t)yt * K32 * 64 + lane) * 8;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            if (s < K32) {
+                const vec yh = *reinterpret_cast<const vec*>(yrow + (int64_t)s * 512);
+                if constexpr (MODE == 1) {
+                    const vec yl = *reinterpret_cast<const vec*>(yrow + (int64_t)s * 512 + ylo_delta);
+#pragma unroll
+                    for (int it = 0; it < 4; ++it) {
+                        acc[it] = mfma16<MODE>(alo[it][s], yh, acc[it]);
+                        acc[it] = mfma16<MODE>(ahi[it][s], yl, acc[it]);
+                        acc[it] = mfma16<MODE>(ahi[it][s], yh, acc[it]);
+                    }
+                } else {
+#pragma unroll
+                    for (int it = 0; it < 4; ++it) acc[it] = mfma16<MODE>(ahi[it][s], yh, acc[it]);
+                }
+            }
+        }
+        const int y = yt * 16 + li;
+        if (y < ny) {
+            float* orow = oz + (int64_t)y * ncols;
+#pragma unroll
+            for (int it = 0; it < 4; ++it) store_tile<VEC>(orow, n0 + 16 * it + 4 * lk, ncols, acc[it]);
+        }
+    }
+}
+
+template <int KMAX, bool VEC>
+static void launch_main(const scw_exec* ex, dim3 grid, hipStream_t st, int ytps, float* out) {
+    const ZMeta* zm = ex->d_zmeta.as<ZMeta>();
+    const float2* U = ex->d_U.as<float2>();
+    const int nx = (int)ex->nx, ny = (int)ex->ny;
+    switch (ex->dtype) {
+        case SCW_DTYPE_FP32:
+            hipLaunchKernelGGL((main_f32<KMAX, VEC>), grid, dim3(256), 0, st, zm, U, ex->d_Yfrag.as<float>(), nx,
+                               ny, ytps, out);
+            break;
+        case SCW_DTYPE_BF16:
+            hipLaunchKernelGGL((main_16<KMAX, VEC, 0>), grid, dim3(256), 0, st, zm, U, ex->d_Yfrag.as<uint16_t>(),
+                               ex->ylo_delta, nx, ny, ytps, out);
+            break;
+        case SCW_DTYPE_BF16X3:
+            hipLaunchKernelGGL((main_16<KMAX, VEC, 1>), grid, dim3(256), 0, st, zm, U, ex->d_Yfrag.as<uint16_t>(),
+                               ex->ylo_delta, nx, ny, ytps, out);
+            break;
+        default:
+            hipLaunchKernelGGL((main_16<KMAX, VEC, 2>), grid, dim3(256), 0, st, zm, U, ex->d_Yfrag.as<uint16_t>(),
+                               ex->ylo_delta, nx, ny, ytps, out);
+            break;
+    }
+}
+
+int launch_gemm(const scw_exec* ex, float* out, hipStream_t st, int stage) {
+    const int nitems = (int)ex->items.size();
+    if (nitems > 0 && (stage == 0 || stage == 1)) {
+        hipLaunchKernelGGL(coef_kernel, dim3((unsigned)((nitems + 3) / 4)), dim3(256), 0, st,
+                           ex->d_items.as<CoefItem>(), nitems, ex->d_regions.as<RegionDev>(), ex->d_zmeta.as<ZMeta>(),
+                           ex->plan->d_ellA.as<float2>(), ex->plan->d_ellC.as<int32_t>(), ex->d_Z.as<float2>(),
+                           ex->c_elems, ex->d_C.as<float2>());
+        SCW_HIP(hipGetLastError());
+    }
+    if (ex->my_max > 0 && (stage == 0 || stage == 2)) {
+        hipLaunchKernelGGL(u_kernel, dim3((unsigned)((ex->nx + 255) / 256), (unsigned)ex->my_max, (unsigned)ex->nz),
+                           dim3(256), 0, st, ex->d_zmeta.as<ZMeta>(), ex->d_regions.as<RegionDev>(),
+                           ex->d_C.as<float2>(), ex->c_elems, ex->d_X.as<float2>(), (int)ex->nx, ex->d_U.as<float2>());
+        SCW_HIP(hipGetLastError());
+    }
+    if (stage != 0 && stage != 3) return SCW_OK;
+    const int64_t ncols = 6 * ex->nx;
+    dim3 grid((unsigned)((ncols + 255) / 256), (unsigned)ex->nz, (unsigned)ex->ysplit);
+    const int ytps = ex->ytiles_per_split;
+    const bool vec = (ncols % 4) == 0;
+#define SCW_MAIN(K)                                                        \
+    if (vec) launch_main<K, true>(ex, grid, st, ytps, out);                \
+    else launch_main<K, false>(ex, grid, st, ytps, out);
+    if (ex->dtype == SCW_DTYPE_FP32) {
+        switch (ex->kmax) {
+            case 16: SCW_MAIN(16) break;
+            case 32: SCW_MAIN(32) break;
+            case 64: SCW_MAIN(64) break;
+            default: SCW_MAIN(128) break;
+        }
+    } else {
+        switch (ex->kmax) {
+            case 32: SCW_MAIN(32) break;
+            case 64: SCW_MAIN(64) break;
+            case 96: SCW_MAIN(96) break;
+            default: SCW_MAIN(128) break;
+        }
+    }
+#undef SCW_MAIN
+    SCW_HIP(hipGetLastError());
+    return SCW_OK;
+}
+
+}  // namespace scw

@@ -1,0 +1,163 @@
+// Internal structures shared by the host planner, the exec builder and the
+// kernel launchers.  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/scwatter_amd.h"
+
+namespace scw {
+
+typedef std::complex<double> cd;
+
+void set_error(const char* fmt, ...);
+int fail(int status, const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what);
+
+#define SCW_HIP(call)                                            \
+    do {                                                         \
+        hipError_t e_ = (call);                                  \
+        if (e_ != hipSuccess) return ::scw::hip_fail(e_, #call); \
+    } while (0)
+
+// Makes `dev` current for the scope and restores the caller's device after.
+struct DeviceGuard {
+    int prev = -1;
+    int st = 0;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        hipError_t e = hipSetDevice(dev);
+        if (e != hipSuccess) st = hip_fail(e, "hipSetDevice");
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+// Device buffer owned by a plan or exec.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int alloc(size_t n);
+    int upload(const void* host, size_t n);
+    void release();
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+// Separable decomposition of one region (the GEMM form's planning result).
+struct RegionPlan {
+    int64_t wbegin = 0, wend = 0;  // wave range in the unpacked table
+    double zref = 0.0;             // FP64 phase reference for kz
+    bool real_transverse = true;   // every wave has Im kx = Im ky = 0
+    // separable form
+    bool separable = false;
+    bool dense = false;            // every cell holds every kz value (eigenmode slabs)
+    int mx = 0, my = 0, mc = 0, width = 0;
+    int64_t n_terms = 0;
+    std::vector<cd> kx, ky, kz;    // distinct values
+    int64_t ell_off = 0;           // slot offset of this region's ELL amplitudes (slot = j*cells + cell)
+    int64_t ellc_off = -1;         // slot offset of its kz indices (-1: dense, index = j)
+    std::string why_not;           // reason when not separable
+};
+
+}  // namespace scw
+
+struct scw_plan {
+    int device = 0;
+    int layers = 0;
+    int64_t n_waves = 0;
+    std::vector<double> zl;
+    std::vector<scw::RegionPlan> regions;
+    // direct-form tables, one entry per wave (unpacked order)
+    scw::DevBuf d_k64;   // double[n][6]  kx, ky, kz (re, im)
+    scw::DevBuf d_e64;   // double[n][6]  E0 / s
+    scw::DevBuf d_lns;   // double[n]     ln s  (s = max_c |E0_c|, 0 -> wave skipped)
+    scw::DevBuf d_kf;    // float4[n]     Re kx/2pi, Re ky/2pi, -Im kx log2e, -Im ky log2e
+    // GEMM-form tables, concatenated over separable regions
+    scw::DevBuf d_ellA;  // float2[...][width][cells][3]  E0 re-referenced to zref
+    scw::DevBuf d_ellC;  // int32[...][width][cells]      kz index (sparse regions only)
+    int64_t ell_elems = 0;
+};
+
+namespace scw {
+
+// Per-z metadata of the GEMM form (device struct).
+struct ZMeta {
+    int32_t region;     // region index
+    int32_t my;         // distinct ky of the region
+    int32_t mx;         // distinct kx of the region
+    int32_t kp;         // padded K of the main GEMM (2*my rounded up to the MFMA K)
+    int64_t u_off;      // float2 offset of U_z ([my][nx][3])
+    int64_t yfrag_off;  // element offset of the region's Y fragment table
+    int64_t zrow_off;   // float2 offset of this z's row of exp(i kz_c (z - zref))
+    int64_t c_off;      // float2 offset of C_z ([cells][3]) within one partial-sum slot
+};
+
+// One wave-sized work item of the coefficient contraction.
+struct CoefItem {
+    int32_t region;
+    int32_t cell0;      // first cell (lane = cell0 + lane id)
+    int32_t j0, j1;     // term slice [j0, j1)
+    int32_t z0, nz;     // first z index in the grid, count (<= 8), all in `region`
+    int32_t slot;       // partial-sum slot
+    int32_t pad;
+};
+
+struct RegionDev {      // per-region data the GEMM-form kernels read (device)
+    int64_t ell_off;    // slot offset in ellA (3 float2 per slot)
+    int64_t ellc_off;   // slot offset in ellC (-1 when dense)
+    int64_t x_off;      // float2 offset of X_r ([mx][nx])
+    int32_t cells, width, mc, dense;
+    int32_t mx, my, nslots, pad;
+};
+
+// Direct form helpers (scw_direct.hip).
+size_t direct_region_bytes();
+void direct_region_fill(void* dst, int64_t begin, int64_t end, int real_transverse);
+
+}  // namespace scw
+
+struct scw_exec {
+    scw_plan* plan = nullptr;
+    int form = SCW_FORM_DIRECT;
+    int dtype = SCW_DTYPE_FP32;
+    int64_t nx = 0, ny = 0, nz = 0;
+    double units = 0, flops = 0, bytes = 0;
+    double stage_flops[4] = {0, 0, 0, 0}, stage_bytes[4] = {0, 0, 0, 0};
+    int launches = 0;
+    std::vector<int> zregion;
+    // direct form
+    double xc = 0, yc = 0;
+    scw::DevBuf d_dx, d_dy;   // float[nx], float[ny]   x - xc, y - yc
+    scw::DevBuf d_z;          // double[nz]
+    scw::DevBuf d_zslab;      // int32[nz]: region per z
+    int direct_ppt = 4;       // points per thread
+    int direct_split = 1;     // wave-range split (partials summed by a second kernel)
+    // GEMM form
+    int kmax = 0;             // largest padded K
+    int my_max = 0;
+    int nslots_max = 1;
+    std::vector<scw::CoefItem> items;
+    scw::DevBuf d_items, d_regions, d_zmeta, d_Z, d_X, d_C, d_U, d_Yfrag;
+    int64_t c_elems = 0;      // float2 per slot of the C buffer
+    int64_t ylo_delta = 0;    // element offset of the lo Y fragments (bf16 x3)
+    int ysplit = 1;           // grid.z of the main GEMM
+    int ytiles_per_split = 1;
+    scw::DevBuf d_out;        // staging for scw_exec_run_host
+    // graph
+    hipGraph_t graph = nullptr;
+    hipStream_t graph_stream = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+};
+
+namespace scw {
+// kernel launchers (scw_direct.hip, scw_gemm.hip)
+int launch_direct(const scw_exec* ex, float* out, hipStream_t s);
+int launch_gemm(const scw_exec* ex, float* out, hipStream_t s, int stage = 0);
+}  // namespace scw

@@ -1,0 +1,403 @@
+// Structure plan: validation, FP64 planning and device upload.
+//
+// Replaces gpu_initialize (gpuEvaluator.cu:15-26), which copied the FP32
+// UnpackedWave table (cpuEvaluator.cpp:61-102) and the region tables
+// (cpuEvaluator.cpp:34-53) to the device.  Here the FP64 waves are kept, and
+// each region is additionally decomposed for the separable GEMM form:
+// distinct kx (a < mx), ky (b < my) and kz (c < mc) values, and terms
+// A[a,b,c] = sum of E0 over the waves with those values, re-referenced in
+// FP64 to the region's phase origin zref.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <array>
+#include <unordered_map>
+#include <new>
+
+#include "scw_internal.hpp"
+
+namespace scw {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+int fail(int status, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return status;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(e == hipErrorOutOfMemory ? SCW_ERR_NOMEM : SCW_ERR_DEVICE, "%s: %s", what,
+                hipGetErrorString(e));
+}
+
+int DevBuf::alloc(size_t n) {
+    release();
+    if (n == 0) return SCW_OK;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+        p = nullptr;
+        return hip_fail(e, "hipMalloc");
+    }
+    bytes = n;
+    return SCW_OK;
+}
+
+int DevBuf::upload(const void* host, size_t n) {
+    int st = alloc(n);
+    if (st != SCW_OK || n == 0) return st;
+    SCW_HIP(hipMemcpy(p, host, n, hipMemcpyHostToDevice));
+    return SCW_OK;
+}
+
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+// Group values that agree to within tol in both parts; returns distinct
+// representatives (first member of each group, in sorted order) and a group
+// index per input.
+static void group_values(const std::vector<cd>& v, double tol, std::vector<cd>& reps, std::vector<int>& idx) {
+    const size_t n = v.size();
+    std::vector<int> order(n);
+    for (size_t i = 0; i < n; ++i) order[i] = (int)i;
+    std::sort(order.begin(), order.end(), [&](int a, int b) {
+        if (v[a].real() != v[b].real()) return v[a].real() < v[b].real();
+        return v[a].imag() < v[b].imag();
+    });
+    reps.clear();
+    idx.assign(n, -1);
+    for (size_t t = 0; t < n; ++t) {
+        const cd x = v[order[t]];
+        bool merged = false;
+        // Look back over representatives whose real part is within tol.
+        for (int g = (int)reps.size() - 1; g >= 0; --g) {
+            if (x.real() - reps[g].real() > tol) break;
+            if (std::fabs(x.imag() - reps[g].imag()) <= tol) {
+                idx[order[t]] = g;
+                merged = true;
+                break;
+            }
+        }
+        if (!merged) {
+            reps.push_back(x);
+            idx[order[t]] = (int)reps.size() - 1;
+        }
+    }
+}
+
+// exp(i*kz*z0 + ln s) * e  — E0 re-referenced without forming the two
+// factors separately (either may overflow alone).
+static inline cd reref(const cd e_hat, double lns, cd kz, double z0) {
+    const cd ph = cd(0, 1) * kz * z0;  // i kz z0
+    const double mag = std::exp(lns + ph.real());
+    return e_hat * cd(mag * std::cos(ph.imag()), mag * std::sin(ph.imag()));
+}
+
+static const int MY_MAX = 64;  // largest ky count of the GEMM form (K = 2*my <= 128)
+
+static void plan_region(RegionPlan& R, const scw_planewave64* W, const std::vector<double>& lns,
+                        const std::vector<cd>& ehat, std::vector<float>& ellA, std::vector<int32_t>& ellC,
+                        int64_t& ellA_slots, int64_t& ellC_slots) {
+    const int64_t n = R.wend - R.wbegin;
+    R.separable = false;
+    R.dense = false;
+    R.ellc_off = -1;
+    if (n == 0) {  // empty region: the field is zero; trivially separable
+        R.separable = true;
+        R.mx = R.my = R.mc = R.width = 0;
+        R.n_terms = 0;
+        R.ell_off = ellA_slots;
+        return;
+    }
+    std::vector<cd> kx(n), ky(n), kz(n);
+    double kscale = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        const scw_planewave64& w = W[R.wbegin + i];
+        kx[i] = cd(w.k[0][0], w.k[0][1]);
+        ky[i] = cd(w.k[1][0], w.k[1][1]);
+        kz[i] = cd(w.k[2][0], w.k[2][1]);
+        kscale = std::max(kscale, std::max(std::abs(kx[i]), std::max(std::abs(ky[i]), std::abs(kz[i]))));
+    }
+    const double tol = 1e-11 * std::max(kscale, 1e-300);
+    std::vector<int> ia, ib, ic;
+    group_values(kx, tol, R.kx, ia);
+    group_values(ky, tol, R.ky, ib);
+    group_values(kz, tol, R.kz, ic);
+    R.mx = (int)R.kx.size();
+    R.my = (int)R.ky.size();
+    R.mc = (int)R.kz.size();
+    const int64_t cells = (int64_t)R.mx * R.my;
+
+    // Terms keyed by (cell, c); E0 summed in FP64 after re-referencing to zref.
+    struct Term {
+        int64_t cell;
+        int c;
+        cd e[3];
+    };
+    std::vector<Term> terms;
+    const int64_t keyspace = cells * (int64_t)R.mc;
+    std::vector<int64_t> slot_of;
+    std::unordered_map<int64_t, int64_t> slot_map;
+    const bool direct_keys = keyspace <= (int64_t(1) << 26);
+    if (direct_keys) slot_of.assign((size_t)keyspace, -1);
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t g = R.wbegin + i;
+        const int64_t cell = (int64_t)ia[i] * R.my + ib[i];
+        const int64_t key = cell * R.mc + ic[i];
+        int64_t t;
+        if (direct_keys) {
+            t = slot_of[(size_t)key];
+            if (t < 0) {
+                t = (int64_t)terms.size();
+                slot_of[(size_t)key] = t;
+                terms.push_back({cell, ic[i], {}});
+            }
+        } else {
+            auto it = slot_map.find(key);
+            if (it == slot_map.end()) {
+                t = (int64_t)terms.size();
+                slot_map[key] = t;
+                terms.push_back({cell, ic[i], {}});
+            } else {
+                t = it->second;
+            }
+        }
+        const cd kzc = R.kz[ic[i]];
+        for (int c = 0; c < 3; ++c) terms[(size_t)t].e[c] += reref(ehat[3 * g + c], lns[g], kzc, R.zref);
+    }
+    // Order terms by (cell, c) so ELL columns hold ascending kz indices.
+    std::sort(terms.begin(), terms.end(),
+              [](const Term& x, const Term& y) { return x.cell != y.cell ? x.cell < y.cell : x.c < y.c; });
+    R.n_terms = (int64_t)terms.size();
+    std::vector<int> count(cells, 0);
+    for (auto& t : terms) count[t.cell]++;
+    int width = 0;
+    for (int64_t c = 0; c < cells; ++c) width = std::max(width, count[c]);
+    R.width = width;
+    bool dense = (int64_t)width == R.mc;
+    for (int64_t c = 0; c < cells && dense; ++c) dense = count[c] == R.mc;
+    R.dense = dense;
+
+    if (R.my > MY_MAX) {
+        R.why_not = "more than 64 distinct ky values";
+        return;
+    }
+    const int64_t slots = (int64_t)width * cells;
+    if (cells > std::max<int64_t>(4 * R.n_terms, 4096) || slots > 4 * R.n_terms + 4096) {
+        R.why_not = "wave set is not a separable kx x ky lattice";
+        return;
+    }
+    for (auto& t : terms)
+        for (int c = 0; c < 3; ++c)
+            if (!std::isfinite(t.e[c].real()) || !std::isfinite(t.e[c].imag()) || std::abs(t.e[c]) > 1e30) {
+                R.why_not = "re-referenced amplitude outside FP32 range";
+                return;
+            }
+    R.separable = true;
+    R.ell_off = ellA_slots;
+    const size_t baseA = ellA.size();
+    ellA.resize(baseA + (size_t)slots * 6, 0.0f);
+    if (!dense) {
+        R.ellc_off = ellC_slots;
+        ellC.resize(ellC.size() + (size_t)slots, 0);
+    }
+    std::vector<int> fill(cells, 0);
+    for (auto& t : terms) {
+        const int j = dense ? t.c : fill[t.cell]++;
+        const int64_t slot = (int64_t)j * cells + t.cell;
+        float* a = &ellA[baseA + (size_t)slot * 6];
+        for (int c = 0; c < 3; ++c) {
+            a[2 * c] = (float)t.e[c].real();
+            a[2 * c + 1] = (float)t.e[c].imag();
+        }
+        if (!dense) ellC[(size_t)(R.ellc_off + slot)] = t.c;
+    }
+    ellA_slots += slots;
+    if (!dense) ellC_slots += slots;
+}
+
+}  // namespace scw
+
+using namespace scw;
+
+extern "C" {
+
+const char* scw_last_error(void) { return g_err; }
+int scw_abi_version(void) { return SCW_ABI_VERSION; }
+
+int scw_device_count(int* count) {
+    if (!count) return fail(SCW_ERR_ARG, "count is NULL");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        *count = 0;
+        return hip_fail(e, "hipGetDeviceCount");
+    }
+    *count = n;
+    return SCW_OK;
+}
+
+int scw_device_mem_info(int device, size_t* free_bytes, size_t* total_bytes) {
+    if (!free_bytes || !total_bytes) return fail(SCW_ERR_ARG, "NULL output");
+    DeviceGuard g(device);
+    if (g.st != SCW_OK) return g.st;
+    SCW_HIP(hipMemGetInfo(free_bytes, total_bytes));
+    return SCW_OK;
+}
+
+int scw_plan_create(int device, const scw_planewave64* waves, size_t n_waves, const double* z_layers,
+                    int layers, const int32_t* waves_begin, const int32_t* waves_end, scw_plan** out) {
+    if (!out) return fail(SCW_ERR_ARG, "plan output is NULL");
+    *out = nullptr;
+    if (layers < 0) return fail(SCW_ERR_ARG, "layers < 0");
+    if (n_waves > 0 && !waves) return fail(SCW_ERR_ARG, "waves is NULL");
+    if (layers > 0 && !z_layers) return fail(SCW_ERR_ARG, "z_layers is NULL");
+    if (!waves_begin || !waves_end) return fail(SCW_ERR_ARG, "waves_begin / waves_end is NULL");
+    if (n_waves > (size_t)INT32_MAX) return fail(SCW_ERR_ARG, "more than 2^31-1 waves");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(SCW_ERR_DEVICE, "no HIP device");
+    if (device < 0 || device >= ndev) return fail(SCW_ERR_ARG, "device %d out of range [0, %d)", device, ndev);
+    for (int l = 0; l < layers; ++l)
+        if (!std::isfinite(z_layers[l])) return fail(SCW_ERR_ARG, "z_layers[%d] is not finite", l);
+    for (int r = 0; r <= layers; ++r) {
+        if (waves_begin[r] < 0 || waves_end[r] < waves_begin[r] || (size_t)waves_end[r] > n_waves)
+            return fail(SCW_ERR_ARG, "region %d wave range [%d, %d) invalid for %zu waves", r, waves_begin[r],
+                        waves_end[r], n_waves);
+    }
+    for (size_t i = 0; i < n_waves; ++i) {
+        const double* v = &waves[i].E0[0][0];
+        for (int j = 0; j < 12; ++j)
+            if (!std::isfinite(v[j])) return fail(SCW_ERR_ARG, "wave %zu has a non-finite value", i);
+    }
+
+    DeviceGuard g(device);
+    if (g.st != SCW_OK) return g.st;
+
+    scw_plan* P = new (std::nothrow) scw_plan();
+    if (!P) return fail(SCW_ERR_NOMEM, "host allocation failed");
+    P->device = device;
+    P->layers = layers;
+    P->n_waves = (int64_t)n_waves;
+    P->zl.assign(z_layers, z_layers + layers);
+    P->regions.resize(layers + 1);
+
+    // Direct-form tables.
+    const double log2e = 1.4426950408889634;
+    const double inv2pi = 0.15915494309189535;
+    std::vector<double> k64(6 * n_waves), e64(6 * n_waves), lns(n_waves);
+    std::vector<float> kf(4 * n_waves);
+    std::vector<cd> ehat(3 * n_waves);
+    for (size_t i = 0; i < n_waves; ++i) {
+        const scw_planewave64& w = waves[i];
+        double s = 0.0;
+        for (int c = 0; c < 3; ++c) s = std::max(s, std::hypot(w.E0[c][0], w.E0[c][1]));
+        lns[i] = s > 0.0 ? std::log(s) : 0.0;
+        for (int c = 0; c < 3; ++c) {
+            ehat[3 * i + c] = s > 0.0 ? cd(w.E0[c][0] / s, w.E0[c][1] / s) : cd(0, 0);
+            e64[6 * i + 2 * c] = ehat[3 * i + c].real();
+            e64[6 * i + 2 * c + 1] = ehat[3 * i + c].imag();
+            k64[6 * i + 2 * c] = w.k[c][0];
+            k64[6 * i + 2 * c + 1] = w.k[c][1];
+        }
+        kf[4 * i + 0] = (float)(w.k[0][0] * inv2pi);
+        kf[4 * i + 1] = (float)(w.k[1][0] * inv2pi);
+        kf[4 * i + 2] = (float)(-w.k[0][1] * log2e);
+        kf[4 * i + 3] = (float)(-w.k[1][1] * log2e);
+    }
+
+    // Region planning.
+    std::vector<float> ellA;
+    std::vector<int32_t> ellC;
+    int64_t slotsA = 0, slotsC = 0;
+    for (int r = 0; r <= layers; ++r) {
+        RegionPlan& R = P->regions[r];
+        R.wbegin = waves_begin[r];
+        R.wend = waves_end[r];
+        if (layers == 0) R.zref = 0.0;
+        else if (r == 0) R.zref = z_layers[0];
+        else if (r == layers) R.zref = z_layers[layers - 1];
+        else R.zref = 0.5 * (z_layers[r - 1] + z_layers[r]);
+        R.real_transverse = true;
+        for (int64_t i = R.wbegin; i < R.wend; ++i)
+            if (waves[i].k[0][1] != 0.0 || waves[i].k[1][1] != 0.0) R.real_transverse = false;
+        try {
+            plan_region(R, waves, lns, ehat, ellA, ellC, slotsA, slotsC);
+        } catch (const std::bad_alloc&) {
+            delete P;
+            return fail(SCW_ERR_NOMEM, "host allocation failed while planning region %d", r);
+        }
+    }
+    P->ell_elems = slotsA;
+
+    int st = SCW_OK;
+    if ((st = P->d_k64.upload(k64.data(), k64.size() * sizeof(double))) != SCW_OK ||
+        (st = P->d_e64.upload(e64.data(), e64.size() * sizeof(double))) != SCW_OK ||
+        (st = P->d_lns.upload(lns.data(), lns.size() * sizeof(double))) != SCW_OK ||
+        (st = P->d_kf.upload(kf.data(), kf.size() * sizeof(float))) != SCW_OK ||
+        (st = P->d_ellA.upload(ellA.data(), ellA.size() * sizeof(float))) != SCW_OK ||
+        (st = P->d_ellC.upload(ellC.data(), ellC.size() * sizeof(int32_t))) != SCW_OK) {
+        delete P;
+        return st;
+    }
+    *out = P;
+    return SCW_OK;
+}
+
+int scw_plan_destroy(scw_plan* plan) {
+    if (!plan) return SCW_OK;
+    {
+        DeviceGuard g(plan->device);
+        plan->d_k64.release();
+        plan->d_e64.release();
+        plan->d_lns.release();
+        plan->d_kf.release();
+        plan->d_ellA.release();
+        plan->d_ellC.release();
+    }
+    delete plan;
+    return SCW_OK;
+}
+
+int scw_plan_layers(const scw_plan* plan, int* layers) {
+    if (!plan || !layers) return fail(SCW_ERR_ARG, "NULL argument");
+    *layers = plan->layers;
+    return SCW_OK;
+}
+
+int scw_plan_region_info(const scw_plan* plan, int region, int64_t* n_waves, int32_t* mx, int32_t* my,
+                         int32_t* mc, int64_t* n_terms, int32_t* width, int32_t* separable) {
+    if (!plan) return fail(SCW_ERR_ARG, "plan is NULL");
+    if (region < 0 || region > plan->layers) return fail(SCW_ERR_ARG, "region %d out of range", region);
+    const RegionPlan& R = plan->regions[region];
+    if (n_waves) *n_waves = R.wend - R.wbegin;
+    if (mx) *mx = R.mx;
+    if (my) *my = R.my;
+    if (mc) *mc = R.mc;
+    if (n_terms) *n_terms = R.n_terms;
+    if (width) *width = R.width;
+    if (separable) *separable = R.separable ? 1 : 0;
+    return SCW_OK;
+}
+
+int scw_plan_device_bytes(const scw_plan* plan, size_t* bytes) {
+    if (!plan || !bytes) return fail(SCW_ERR_ARG, "NULL argument");
+    *bytes = plan->d_k64.bytes + plan->d_e64.bytes + plan->d_lns.bytes + plan->d_kf.bytes +
+             plan->d_ellA.bytes + plan->d_ellC.bytes;
+    return SCW_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,241 @@
+"""GPU parity: the HIP kernels, called through the C ABI, against the FP64
+oracle and the mpmath golden vectors.
+
+Tolerances (relative L2 over all three components, and max-abs relative to
+the field's max):
+  direct FP32          rel-L2 <= 1e-5   (config-1 class fields, |k.r| <= ~150)
+  GEMM FP32, BF16 x3   rel-L2 <= 1e-5 / 1e-4
+  GEMM BF16, FP16      rel-L2 <= 1e-2 (reported precision forms, not the bar)
+"""
+import numpy as np
+import pytest
+
+import oracle
+from scwatter_amd import Exec, Plan, configs, cpu_cw_evaluate, gpu_cw_evaluate
+from scwatter_amd.cw import CoupledWaveStructure
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp32": 1e-5, "bf16x3": 1e-4, "bf16": 1e-2, "fp16": 1e-2}
+
+
+def _log(msg):
+    print(msg, flush=True)
+
+
+def _check(E, ref, tol, what):
+    rl2 = oracle.rel_l2(E, ref)
+    mx = oracle.max_abs(E, ref) / max(np.abs(ref).max(), 1e-30)
+    _log(f"{what}: rel-L2 {rl2:.3e} max-abs/max {mx:.3e}")
+    assert np.isfinite(E).all(), what
+    assert rl2 <= tol, f"{what}: rel-L2 {rl2:.3e} > {tol:.1e}"
+    return rl2
+
+
+@pytest.mark.parametrize("case", ["A", "B", "C", "D"])
+def test_direct_matches_mpmath(golden, case):
+    g = golden[case]
+    _log(f"golden case {case}")
+    from scwatter_amd.cw import WaveTables
+    plan = Plan(WaveTables(g["W"], g["zl"], g["begin"], g["end"]))
+    ex = Exec(plan, g["xs"], g["ys"], g["zs"], float(g["tol"]), form="direct")
+    E = ex.run_host()
+    _check(E, g["E"], 1e-5, f"direct golden {case}")
+
+
+def test_direct_matches_mpmath_points(golden):
+    g = golden["E"]
+    from scwatter_amd.cw import WaveTables
+    plan = Plan(WaveTables(g["W"], g["zl"], g["begin"], g["end"]))
+    out = []
+    for p in g["pts"]:
+        out.append(Exec(plan, [p[0]], [p[1]], [p[2]], 0.0, form="direct").run_host().reshape(3))
+    _check(np.array(out), g["E"], 1e-5, "direct golden E (points)")
+
+
+@pytest.fixture(scope="module")
+def c1():
+    cw, (xs, ys, zs) = configs.config1()
+    t = cw.unpack()
+    ref = oracle.eval_grid_f64(t.W, t.z_layers, t.begin, t.end, xs, ys, zs)
+    return Plan(t), (xs, ys, zs), ref
+
+
+def test_c1_direct_full_grid(c1):
+    plan, g, ref = c1
+    ex = Exec(plan, *g, form="direct")
+    assert ex.info()["form"] == "direct"
+    _check(ex.run_host(), ref, 1e-5, "C1 direct 64^3")
+
+
+@pytest.mark.parametrize("dtype", ["fp32", "bf16x3", "bf16", "fp16"])
+def test_c1_gemm_full_grid(c1, dtype):
+    plan, g, ref = c1
+    for r in range(2):
+        assert plan.region_info(r)["separable"]
+    ex = Exec(plan, *g, form="gemm", dtype=dtype)
+    info = ex.info()
+    assert info["form"] == "gemm" and info["dtype"] == dtype
+    _check(ex.run_host(), ref, TOL[dtype], f"C1 gemm {dtype} 64^3")
+
+
+def test_c1_auto_picks_gemm(c1):
+    plan, g, _ = c1
+    assert Exec(plan, *g).info()["form"] == "gemm"
+
+
+def test_c1_units(c1):
+    plan, (xs, ys, zs), _ = c1
+    info = Exec(plan, xs, ys, zs, form="direct").info()
+    n_below = int((zs < 0).sum())
+    assert info["units"] == 64 * 64 * (n_below * 882 + (64 - n_below) * 441)
+
+
+@pytest.fixture(scope="module")
+def tile_small():
+    cw = configs.tile_sample_structure(M=5, Z=4, size=(16.0, 16.0, 4.0), lam=1.0, seed=7)
+    t = cw.unpack()
+    xs, ys, zs = configs.axis(-8, 8, 24), configs.axis(-8, 8, 20), configs.axis(-3, 3, 26)
+    ref = oracle.eval_grid_f64(t.W, t.z_layers, t.begin, t.end, xs, ys, zs)
+    return Plan(t), (xs, ys, zs), ref
+
+
+def test_tile_sample_direct(tile_small):
+    plan, g, ref = tile_small
+    _check(Exec(plan, *g, form="direct").run_host(), ref, 1e-5, "tile sample direct")
+
+
+@pytest.mark.parametrize("dtype", ["fp32", "bf16x3"])
+def test_tile_sample_gemm(tile_small, dtype):
+    plan, g, ref = tile_small
+    info = plan.region_info(2)
+    assert info["separable"] and info["n_waves"] == 4 * 25 * 25 and info["mc"] == 100 and info["width"] == 100
+    _check(Exec(plan, *g, form="gemm", dtype=dtype).run_host(), ref, TOL[dtype], f"tile sample gemm {dtype}")
+
+
+@pytest.fixture(scope="module")
+def stack_small():
+    rng = np.random.default_rng(11)
+    n = np.concatenate([[1.0], rng.uniform(1, 2, 5) + 1j * rng.uniform(0, 0.05, 5)])
+    z = np.concatenate([[0.0], np.cumsum(rng.uniform(0.1, 2.0, 4))]) - 2.0
+    cw = configs.layered_structure(n, z, 0.6, 0.7, 11, ("x",))
+    t = cw.unpack()
+    xs, ys, zs = configs.axis(-4, 4, 20), configs.axis(-4, 4, 18), configs.axis(z[0] - 1, z[-1] + 1, 40)
+    ref = oracle.eval_grid_f64(t.W, t.z_layers, t.begin, t.end, xs, ys, zs)
+    return Plan(t), (xs, ys, zs), ref
+
+
+@pytest.mark.parametrize("form,dtype", [("direct", "fp32"), ("gemm", "fp32"), ("gemm", "bf16x3")])
+def test_layer_stack(stack_small, form, dtype):
+    plan, g, ref = stack_small
+    _check(Exec(plan, *g, form=form, dtype=dtype).run_host(), ref, TOL[dtype] if form == "gemm" else 1e-5,
+           f"layer stack {form} {dtype}")
+
+
+def test_gpu_cw_evaluate_planes(c1):
+    """gpu_cw_evaluate semantics: fused coordinates, region rule z >= z_l - 0.001."""
+    plan, _, _ = c1
+    t = plan.tables
+    N, d = 32, np.float32(0.31)
+    xs0, ys0, zs0 = np.float32(-4.8), np.float32(-5.1), np.float32(-4.9)
+    x, y, z = 0.2, -0.4, -0.0005  # z within 1e-3 below the interface: region 1 under the GPU rule
+    Exy, Exz, Eyz = gpu_cw_evaluate(plan, xs0, ys0, zs0, x, y, z, d, N, form="direct")
+    from scwatter_amd.evaluator import axis_gpu
+    ax, ay, az = axis_gpu(xs0, d, N), axis_gpu(ys0, d, N), axis_gpu(zs0, d, N)
+    zl = t.z_layers.astype(np.float32).astype(np.float64)
+    f = lambda v: [float(np.float32(v))]
+    for E, grid, name in ((Exy, (ax, ay, f(z)), "xy"), (Exz, (ax, f(y), az), "xz"), (Eyz, (f(x), ay, az), "yz")):
+        ref = oracle.eval_grid_f64(t.W, zl, t.begin, t.end, *grid, tol=0.001).reshape(N, N, 3)
+        _check(E, ref, 1e-5, f"gpu_cw_evaluate {name}")
+
+
+@pytest.mark.parametrize("kind", ["xy", "yz", "xz"])
+def test_cpu_cw_evaluate_planes_match_reference_semantics(c1, kind):
+    """cpu_cw_evaluate_* names run on the GPU; results agree with the FP32 reference-semantics port."""
+    plan, _, _ = c1
+    t = plan.tables
+    N, d, a0, b0, fixed = 24, 0.41, -5.0, -4.7, -0.0005  # fixed z below the interface (CPU rule: region 0)
+    E = cpu_cw_evaluate(kind, plan, a0, b0, fixed, d, N, form="direct")
+    R = oracle.refcpu_plane(kind, t.W, t.z_layers, t.begin, t.end, a0, b0, fixed, d, N)
+    _check(E, R, 2e-5, f"cpu_cw_evaluate_{kind} vs reference-semantics port")
+
+
+def test_gemm_plane_equals_volume_slice(c1):
+    plan, (xs, ys, zs), _ = c1
+    V = Exec(plan, xs, ys, zs, form="gemm").run_host()
+    for iz in (0, 31, 63):
+        P = Exec(plan, xs, ys, [zs[iz]], form="gemm").run_host()[0]
+        assert np.array_equal(P, V[iz]), f"xy plane at iz={iz} differs from the volume slice"
+
+
+def test_direct_plane_matches_volume_slice(c1):
+    plan, (xs, ys, zs), _ = c1
+    V = Exec(plan, xs, ys, zs, form="direct").run_host()
+    P = Exec(plan, xs, [ys[5]], zs, form="direct").run_host()[:, 0]
+    assert oracle.rel_l2(P, V[:, 5]) < 1e-6
+
+
+def test_empty_region_gives_zero_field():
+    W = np.array([[1, 0, 0, 0.3, 0.1, 1.0]], np.complex128)
+    from scwatter_amd.cw import WaveTables
+    plan = Plan(WaveTables(W, np.array([0.0]), np.array([0, 1], np.int32), np.array([1, 1], np.int32)))
+    for form in ("direct", "gemm"):
+        E = Exec(plan, [0.0, 1.0], [0.0], [0.5, 1.5], form=form).run_host()
+        assert np.all(E == 0), form
+
+
+def test_output_is_overwritten_not_accumulated(c1):
+    from scwatter_amd import hip
+    plan, g, ref = c1
+    ex = Exec(plan, *g, form="gemm")
+    buf = hip.DeviceArray(ex.shape, np.complex64)
+    garbage = np.full(ex.shape, 1e3 + 1e3j, np.complex64)
+    buf2 = hip.DeviceArray.from_host(garbage)
+    ex.run(buf2.ptr)
+    ex.run(buf.ptr)
+    hip.synchronize()
+    assert np.array_equal(buf.to_host(), buf2.to_host())
+
+
+def test_graph_replay_matches_eager(c1):
+    from scwatter_amd import hip
+    plan, g, _ = c1
+    for form in ("direct", "gemm"):
+        ex = Exec(plan, *g, form=form)
+        eager = ex.run_host()
+        s = hip.Stream()
+        buf = hip.DeviceArray(ex.shape, np.complex64)
+        ex.graph_capture(buf.ptr, s.handle)
+        for _ in range(3):
+            hip.DeviceArray.from_host(np.zeros(4, np.float32))  # eager work between replays
+            ex.graph_launch(s.handle)
+            s.synchronize()
+            assert np.array_equal(buf.to_host(), eager), form
+        s.destroy()
+
+
+def test_errors_are_reported():
+    from scwatter_amd import ScwError
+    from scwatter_amd.cw import WaveTables
+    rng = np.random.default_rng(5)
+    # polar-like pupil: every wave has its own kx and ky -> not a separable lattice
+    n = 600
+    W = np.column_stack([np.ones(n), np.zeros(n), np.zeros(n), rng.uniform(-1, 1, n), rng.uniform(-1, 1, n),
+                         np.ones(n)])
+    plan = Plan(WaveTables(W, np.zeros(0), np.array([0], np.int32), np.array([n], np.int32)))
+    assert not plan.region_info(0)["separable"]
+    with pytest.raises(ScwError, match="UNSUPPORTED"):
+        Exec(plan, [0.0], [0.0], [0.0], form="gemm")
+    assert Exec(plan, [0.0], [0.0], [0.0]).info()["form"] == "direct"
+    with pytest.raises(ScwError, match="ARG"):
+        Exec(plan, [], [0.0], [0.0])
+    with pytest.raises(ScwError, match="ARG"):
+        Exec(plan, [np.nan], [0.0], [0.0])
+    bad = W.copy()
+    bad[3, 0] = np.inf
+    with pytest.raises(ScwError, match="ARG"):
+        Plan(WaveTables(bad, np.zeros(0), np.array([0], np.int32), np.array([n], np.int32)))
+    with pytest.raises(ScwError, match="ARG"):
+        Plan(WaveTables(W, np.zeros(0), np.array([0], np.int32), np.array([n + 1], np.int32)))
+    with pytest.raises(ScwError, match="ARG"):
+        Plan(WaveTables(W, np.zeros(0), np.array([0], np.int32), np.array([n], np.int32)), device=99)
