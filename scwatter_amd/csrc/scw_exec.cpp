@@ -265,7 +265,8 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const dou
     if ((st = ex->d_C.alloc((size_t)ex->nslots_max * std::max<int64_t>(c_off, 1) * 8)) != SCW_OK) return st;
     if ((st = ex->d_U.alloc((size_t)std::max<int64_t>(u_off, 1) * 8)) != SCW_OK) return st;
     if (!is16) {
-        if ((st = ex->d_Yfrag.upload(yf32.data(), std::max<size_t>(yf32.size(), 1) * 4)) != SCW_OK) return st;
+        if (yf32.empty()) yf32.push_back(0.0f);  // every region on the grid is empty: keep a valid pointer
+        if ((st = ex->d_Yfrag.upload(yf32.data(), yf32.size() * 4)) != SCW_OK) return st;
         ex->ylo_delta = 0;
     } else {
         std::vector<uint16_t> both(yhi);

@@ -55,6 +55,7 @@ int DevBuf::alloc(size_t n) {
 }
 
 int DevBuf::upload(const void* host, size_t n) {
+    if (n > 0 && !host) return fail(SCW_ERR_ARG, "internal: upload of %zu bytes from NULL", n);
     int st = alloc(n);
     if (st != SCW_OK || n == 0) return st;
     SCW_HIP(hipMemcpy(p, host, n, hipMemcpyHostToDevice));

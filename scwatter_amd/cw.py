@@ -45,6 +45,12 @@ class WaveTables:
     begin: np.ndarray
     end: np.ndarray
 
+    def __post_init__(self):
+        self.W = _waves(self.W)
+        self.z_layers = np.ascontiguousarray(self.z_layers, np.float64).reshape(-1)
+        self.begin = np.ascontiguousarray(self.begin, np.int32).reshape(-1)
+        self.end = np.ascontiguousarray(self.end, np.int32).reshape(-1)
+
     @property
     def layers(self):
         return int(self.z_layers.shape[0])

@@ -172,7 +172,7 @@ def test_direct_plane_matches_volume_slice(c1):
     plan, (xs, ys, zs), _ = c1
     V = Exec(plan, xs, ys, zs, form="direct").run_host()
     P = Exec(plan, xs, [ys[5]], zs, form="direct").run_host()[:, 0]
-    assert oracle.rel_l2(P, V[:, 5]) < 1e-6
+    assert oracle.rel_l2(P, V[:, 5]) < 1e-5  # both sit within FP32 rounding of the oracle
 
 
 def test_empty_region_gives_zero_field():
