@@ -117,8 +117,7 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const dou
         d.dense = R.dense ? 1 : 0;
         d.mx = R.mx;
         d.my = R.my;
-        d.nslots = R.width <= 128 ? 1 : std::min(16, (R.width + 95) / 96);
-        ex->nslots_max = std::max(ex->nslots_max, d.nslots);
+        d.nslots = 1;  // set below, once the number of work items is known
         ex->my_max = std::max(ex->my_max, R.my);
         // X_r[a][x] = exp(i kx_a x)
         d.x_off = (int64_t)X.size() / 2;
@@ -176,6 +175,25 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const dou
     }
     if (kmax > 128) return fail(SCW_ERR_UNSUPPORTED, "GEMM form supports at most 64 distinct ky per region");
 
+    // Term-slice split: enough wave-sized work items to cover every SIMD a few
+    // times over, with at least 16 terms per slice.
+    {
+        int64_t base = 0;
+        for (int z = 0; z < nz;) {
+            const int r = ex->zregion[z];
+            int e = z + 1;
+            while (e < nz && e - z < 8 && ex->zregion[e] == r) ++e;
+            base += (rd[r].cells + 63) / 64;
+            z = e;
+        }
+        const int want = base > 0 ? (int)std::min<int64_t>(64, (4096 + base - 1) / base) : 1;
+        for (int r = 0; r <= L; ++r)
+            if (used[r]) {
+                rd[r].nslots = std::max(1, std::min(want, rd[r].width / 16));
+                ex->nslots_max = std::max(ex->nslots_max, rd[r].nslots);
+            }
+    }
+
     // Per-z metadata, Z rows, C and U offsets.
     std::vector<ZMeta> zm(nz);
     std::vector<float> Z;
@@ -209,8 +227,8 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const dou
         f_main += 24.0 * (double)R.my * nx * ny;
         const double cbytes = 24.0 * (double)R.mx * R.my;   // C_z, one slot
         const double ubytes = 24.0 * (double)R.my * nx;      // U_z
-        b_coef += cbytes * (R.width <= 128 ? 1 : std::min(16, (R.width + 95) / 96));
-        b_u += cbytes + ubytes + 8.0 * R.mx * nx;
+        b_coef += cbytes * rd[r].nslots;
+        b_u += cbytes * rd[r].nslots + ubytes + 8.0 * R.mx * nx;
         b_main += ubytes;
     }
     ex->c_elems = c_off;
@@ -252,7 +270,10 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const dou
     ex->launches = (ex->items.empty() ? 0 : 1) + (ex->my_max > 0 ? 1 : 0) + 1;
 
     const int gx = (int)((6LL * nx + 255) / 256);
-    int ysplit = (int)std::min<int64_t>(ytiles, std::max<int64_t>(1, (2048 + (int64_t)gx * nz - 1) / ((int64_t)gx * nz)));
+    // Each wave keeps its B-operand fragment (from U_z) in registers for all of
+    // its y tiles, so split y across workgroups only as far as needed to give
+    // every CU a couple of workgroups.
+    int ysplit = (int)std::min<int64_t>(ytiles, std::max<int64_t>(1, (512 + (int64_t)gx * nz - 1) / ((int64_t)gx * nz)));
     ex->ytiles_per_split = (ytiles + ysplit - 1) / ysplit;
     ex->ysplit = (ytiles + ex->ytiles_per_split - 1) / ex->ytiles_per_split;
 

@@ -33,51 +33,107 @@ typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
 
 // ---------------------------------------------------------------------------
 // C_z[a,b,comp] partial sums over a term slice, for up to 8 z of one region.
+// One wave per work item; the item, region and Z-row offsets are wave-uniform
+// (scalar registers), so for dense eigenmode regions (kz index = term index)
+// the Z values are scalar loads.  Terms are consumed four at a time with all
+// twelve amplitude loads issued before the arithmetic, to keep several
+// HBM requests in flight per wave.
 // ---------------------------------------------------------------------------
+__device__ inline void cmac6(float* acc, const float2 a0, const float2 a1, const float2 a2, const float2 f) {
+    acc[0] = fmaf(a0.x, f.x, fmaf(-a0.y, f.y, acc[0]));
+    acc[1] = fmaf(a0.x, f.y, fmaf(a0.y, f.x, acc[1]));
+    acc[2] = fmaf(a1.x, f.x, fmaf(-a1.y, f.y, acc[2]));
+    acc[3] = fmaf(a1.x, f.y, fmaf(a1.y, f.x, acc[3]));
+    acc[4] = fmaf(a2.x, f.x, fmaf(-a2.y, f.y, acc[4]));
+    acc[5] = fmaf(a2.x, f.y, fmaf(a2.y, f.x, acc[5]));
+}
+
+// Contraction of one work item for exactly NZ z rows.  DENSE: the kz index
+// of term j is j, so the Z address is wave-uniform and Z is read through the
+// scalar cache.
+template <int NZ, bool DENSE>
+__device__ inline void coef_loop(float (&acc)[8][6], const float2* __restrict__ Ab,
+                                 const int32_t* __restrict__ Cb, int64_t stride, const float2* __restrict__ Z,
+                                 const int64_t (&zrow)[8], int j0, int j1) {
+    int j = j0;
+    for (; j + 4 <= j1; j += 4) {
+        float2 a[4][3];
+        int c[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const float2* p = Ab + (int64_t)(j + u) * stride * 3;
+            a[u][0] = p[0];
+            a[u][1] = p[1];
+            a[u][2] = p[2];
+            c[u] = DENSE ? j + u : Cb[(int64_t)(j + u) * stride];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            float2 f[NZ];
+#pragma unroll
+            for (int zi = 0; zi < NZ; ++zi) f[zi] = Z[zrow[zi] + c[u]];
+#pragma unroll
+            for (int zi = 0; zi < NZ; ++zi) cmac6(acc[zi], a[u][0], a[u][1], a[u][2], f[zi]);
+        }
+    }
+    for (; j < j1; ++j) {
+        const float2* p = Ab + (int64_t)j * stride * 3;
+        const float2 a0 = p[0], a1 = p[1], a2 = p[2];
+        const int c = DENSE ? j : Cb[(int64_t)j * stride];
+#pragma unroll
+        for (int zi = 0; zi < NZ; ++zi) cmac6(acc[zi], a0, a1, a2, Z[zrow[zi] + c]);
+    }
+}
+
 __global__ __launch_bounds__(256) void coef_kernel(const CoefItem* __restrict__ items, int nitems,
                                                    const RegionDev* __restrict__ regions,
                                                    const ZMeta* __restrict__ zmeta, const float2* __restrict__ ellA,
                                                    const int32_t* __restrict__ ellC, const float2* __restrict__ Z,
                                                    int64_t c_elems, float2* __restrict__ C) {
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int it = blockIdx.x * 4 + wv;
+    const int lane = threadIdx.x & 63;
+    const int it = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
     if (it >= nitems) return;
     const CoefItem I = items[it];
     const RegionDev R = regions[I.region];
     const int cell = I.cell0 + lane;
     const bool active = cell < R.cells;
     const int cl = active ? cell : 0;
+    const int nz = I.nz;
     int64_t zrow[8];
 #pragma unroll
-    for (int zi = 0; zi < 8; ++zi) zrow[zi] = zmeta[I.z0 + min(zi, I.nz - 1)].zrow_off;
+    for (int zi = 0; zi < 8; ++zi) zrow[zi] = zmeta[I.z0 + min(zi, nz - 1)].zrow_off;
     float acc[8][6];
 #pragma unroll
     for (int zi = 0; zi < 8; ++zi)
 #pragma unroll
         for (int c = 0; c < 6; ++c) acc[zi][c] = 0.0f;
 
-    for (int j = I.j0; j < I.j1; ++j) {
-        const int64_t slot = (int64_t)j * R.cells + cl;
-        const float2* a = ellA + (R.ell_off + slot) * 3;
-        const float2 a0 = a[0], a1 = a[1], a2 = a[2];
-        const int c = R.dense ? j : ellC[R.ellc_off + slot];
-#pragma unroll
-        for (int zi = 0; zi < 8; ++zi) {
-            if (zi < I.nz) {
-                const float2 f = Z[zrow[zi] + c];
-                acc[zi][0] = fmaf(a0.x, f.x, fmaf(-a0.y, f.y, acc[zi][0]));
-                acc[zi][1] = fmaf(a0.x, f.y, fmaf(a0.y, f.x, acc[zi][1]));
-                acc[zi][2] = fmaf(a1.x, f.x, fmaf(-a1.y, f.y, acc[zi][2]));
-                acc[zi][3] = fmaf(a1.x, f.y, fmaf(a1.y, f.x, acc[zi][3]));
-                acc[zi][4] = fmaf(a2.x, f.x, fmaf(-a2.y, f.y, acc[zi][4]));
-                acc[zi][5] = fmaf(a2.x, f.y, fmaf(a2.y, f.x, acc[zi][5]));
-            }
-        }
+    const float2* Ab = ellA + (R.ell_off + cl) * 3;
+    const int32_t* Cb = ellC + R.ellc_off + cl;
+    const int64_t stride = (int64_t)R.cells;
+#define SCW_COEF_CASE(N)                                                                   \
+    case N:                                                                                \
+        if (R.dense) coef_loop<N, true>(acc, Ab, Cb, stride, Z, zrow, I.j0, I.j1);        \
+        else coef_loop<N, false>(acc, Ab, Cb, stride, Z, zrow, I.j0, I.j1);               \
+        break;
+    switch (nz) {
+        SCW_COEF_CASE(1)
+        SCW_COEF_CASE(2)
+        SCW_COEF_CASE(3)
+        SCW_COEF_CASE(4)
+        SCW_COEF_CASE(5)
+        SCW_COEF_CASE(6)
+        SCW_COEF_CASE(7)
+        default:
+            if (R.dense) coef_loop<8, true>(acc, Ab, Cb, stride, Z, zrow, I.j0, I.j1);
+            else coef_loop<8, false>(acc, Ab, Cb, stride, Z, zrow, I.j0, I.j1);
+            break;
     }
+#undef SCW_COEF_CASE
     if (!active) return;
 #pragma unroll
     for (int zi = 0; zi < 8; ++zi) {
-        if (zi < I.nz) {
+        if (zi < nz) {
             float2* o = C + (int64_t)I.slot * c_elems + zmeta[I.z0 + zi].c_off + (int64_t)cell * 3;
             o[0] = make_float2(acc[zi][0], acc[zi][1]);
             o[1] = make_float2(acc[zi][2], acc[zi][3]);
@@ -88,35 +144,60 @@ __global__ __launch_bounds__(256) void coef_kernel(const CoefItem* __restrict__ 
 
 // ---------------------------------------------------------------------------
 // U_z[b][x][comp] = sum_a (sum_slots C_z[a,b,comp]) X[a][x]
+// The block (z, b) first adds the partial-sum slots of C_z[:, b, :] into LDS
+// (fixed slot order), then each thread contracts one x against them.
 // ---------------------------------------------------------------------------
+constexpr int MX_MAX = 128;
+
 __global__ __launch_bounds__(256) void u_kernel(const ZMeta* __restrict__ zmeta, const RegionDev* __restrict__ regions,
                                                 const float2* __restrict__ C, int64_t c_elems,
                                                 const float2* __restrict__ X, int nx, float2* __restrict__ U) {
+    __shared__ float2 sc[MX_MAX * 3];
+    __shared__ float2 part[256];
     const int z = blockIdx.z, b = blockIdx.y;
     const ZMeta zm = zmeta[z];
     if (b >= zm.my) return;
+    const RegionDev R = regions[zm.region];
+    // Sum the partial-sum slots of C_z[:, b, :]: entry t = (a, comp) of m = 3 mx
+    // entries; when m < 128 the threads form G = 256 / m groups that each add
+    // every G-th slot, and the groups are then added in group order.
+    const int m = R.mx * 3;
+    const int G = m < 128 ? 256 / m : 1;
+    for (int t0 = 0; t0 < m; t0 += 256) {
+        const int t = t0 + (int)threadIdx.x % (G > 1 ? m : 256);
+        const int g = G > 1 ? (int)threadIdx.x / m : 0;
+        float2 s = make_float2(0.0f, 0.0f);
+        if (t < m && g < G) {
+            const int64_t off = zm.c_off + ((int64_t)b * R.mx + t / 3) * 3 + (t % 3);
+            for (int k = g; k < R.nslots; k += G) {
+                const float2 v = C[(int64_t)k * c_elems + off];
+                s.x += v.x;
+                s.y += v.y;
+            }
+        }
+        if (G > 1) {
+            if (g < G) part[threadIdx.x] = s;
+            __syncthreads();
+            if ((int)threadIdx.x < m) {
+                float2 tot = part[threadIdx.x];
+                for (int q = 1; q < G; ++q) {
+                    const float2 v = part[q * m + threadIdx.x];
+                    tot.x += v.x;
+                    tot.y += v.y;
+                }
+                sc[threadIdx.x] = tot;
+            }
+        } else if (t < m) {
+            sc[t] = s;
+        }
+    }
+    __syncthreads();
     const int x = blockIdx.x * 256 + threadIdx.x;
     if (x >= nx) return;
-    const RegionDev R = regions[zm.region];
     float acc[6] = {0, 0, 0, 0, 0, 0};
-    for (int a = 0; a < R.mx; ++a) {
-        const float2 xv = X[R.x_off + (int64_t)a * nx + x];
-        const int64_t cb = zm.c_off + ((int64_t)a * R.my + b) * 3;
-        float2 c0 = make_float2(0, 0), c1 = c0, c2 = c0;
-        for (int s = 0; s < R.nslots; ++s) {
-            const float2* cs = C + (int64_t)s * c_elems + cb;
-            const float2 t0 = cs[0], t1 = cs[1], t2 = cs[2];
-            c0.x += t0.x; c0.y += t0.y;
-            c1.x += t1.x; c1.y += t1.y;
-            c2.x += t2.x; c2.y += t2.y;
-        }
-        acc[0] = fmaf(c0.x, xv.x, fmaf(-c0.y, xv.y, acc[0]));
-        acc[1] = fmaf(c0.x, xv.y, fmaf(c0.y, xv.x, acc[1]));
-        acc[2] = fmaf(c1.x, xv.x, fmaf(-c1.y, xv.y, acc[2]));
-        acc[3] = fmaf(c1.x, xv.y, fmaf(c1.y, xv.x, acc[3]));
-        acc[4] = fmaf(c2.x, xv.x, fmaf(-c2.y, xv.y, acc[4]));
-        acc[5] = fmaf(c2.x, xv.y, fmaf(c2.y, xv.x, acc[5]));
-    }
+    const float2* Xr = X + R.x_off + x;
+    for (int a = 0; a < R.mx; ++a)
+        cmac6(acc, sc[3 * a], sc[3 * a + 1], sc[3 * a + 2], Xr[(int64_t)a * nx]);
     float2* o = U + zm.u_off + ((int64_t)b * nx + x) * 3;
     o[0] = make_float2(acc[0], acc[1]);
     o[1] = make_float2(acc[2], acc[3]);
@@ -130,15 +211,38 @@ __device__ inline float bp_value(const float* __restrict__ Ub, int64_t ncols, in
     return (k & 1) == 0 ? (odd ? u.y : u.x) : (odd ? u.x : -u.y);
 }
 
+// Epilogue of one 16-row y tile of a workgroup (256 output columns): every
+// wave parks its 16 x 64 accumulator tile in LDS, then each store
+// instruction writes one full 1 KB row segment (lane l: columns 4l..4l+3).
+// The field is written once and not read back, so the stores are
+// non-temporal.
+constexpr int TROW = 256 + 4;
+
 template <bool VEC>
-__device__ inline void store_tile(float* __restrict__ orow, int64_t n, int64_t ncols, const floatx4 v) {
-    if (VEC) {
-        if (n < ncols) *reinterpret_cast<float4*>(orow + n) = make_float4(v[0], v[1], v[2], v[3]);
-    } else {
+__device__ inline void tile_out(float (*T)[TROW], const floatx4 (&acc)[4], float* __restrict__ oz, int64_t ncols,
+                                int ny, int yt, int64_t nwg0) {
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-            if (n + r < ncols) orow[n + r] = v[r];
+    for (int it = 0; it < 4; ++it) *reinterpret_cast<floatx4*>(&T[li][64 * wv + 16 * it + 4 * lk]) = acc[it];
+    __syncthreads();
+    const int64_t n = nwg0 + 4 * lane;
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+        const int row = 4 * rr + wv;
+        const int y = yt * 16 + row;
+        if (y < ny) {
+            const floatx4 v = *reinterpret_cast<const floatx4*>(&T[row][4 * lane]);
+            float* orow = oz + (int64_t)y * ncols;
+            if (VEC) {
+                if (n < ncols) __builtin_nontemporal_store(v, reinterpret_cast<floatx4*>(orow + n));
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (n + r < ncols) __builtin_nontemporal_store(v[r], orow + n + r);
+            }
+        }
     }
+    __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
@@ -155,10 +259,10 @@ __global__ __launch_bounds__(256) void main_f32(const ZMeta* __restrict__ zmeta,
     const int K4 = zm.kp >> 2;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t ncols = 6LL * nx;
-    const int64_t n0 = (int64_t)blockIdx.x * 256 + wv * 64;
-    if (n0 >= ncols) return;
+    const int64_t n0 = (int64_t)blockIdx.x * 256 + wv * 64;  // waves past ncols compute zeros
     const int li = lane & 15, lk = lane >> 4;
     const float* Ub = reinterpret_cast<const float*>(U + zm.u_off);
+    __shared__ float T[16][TROW];
 
     float uf[4][KMAX / 4];
 #pragma unroll
@@ -190,12 +294,7 @@ __global__ __launch_bounds__(256) void main_f32(const ZMeta* __restrict__ zmeta,
                     acc[it] = __builtin_amdgcn_mfma_f32_16x16x4f32(uf[it][s], yv, acc[it], 0, 0, 0);
             }
         }
-        const int y = yt * 16 + li;
-        if (y < ny) {
-            float* orow = oz + (int64_t)y * ncols;
-#pragma unroll
-            for (int it = 0; it < 4; ++it) store_tile<VEC>(orow, n0 + 16 * it + 4 * lk, ncols, acc[it]);
-        }
+        tile_out<VEC>(T, acc, oz, ncols, ny, yt, (int64_t)blockIdx.x * 256);
     }
 }
 
@@ -244,10 +343,10 @@ __global__ __launch_bounds__(256) void main_16(const ZMeta* __restrict__ zmeta, 
     const int K32 = zm.kp >> 5;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t ncols = 6LL * nx;
-    const int64_t n0 = (int64_t)blockIdx.x * 256 + wv * 64;
-    if (n0 >= ncols) return;
+    const int64_t n0 = (int64_t)blockIdx.x * 256 + wv * 64;  // waves past ncols compute zeros
     const int li = lane & 15, lk = lane >> 4;
     const float* Ub = reinterpret_cast<const float*>(U + zm.u_off);
+    __shared__ float T[16][TROW];
 
     vec ahi[4][S], alo[4][S];
 #pragma unroll
@@ -294,12 +393,7 @@ __global__ __launch_bounds__(256) void main_16(const ZMeta* __restrict__ zmeta, 
                 }
             }
         }
-        const int y = yt * 16 + li;
-        if (y < ny) {
-            float* orow = oz + (int64_t)y * ncols;
-#pragma unroll
-            for (int it = 0; it < 4; ++it) store_tile<VEC>(orow, n0 + 16 * it + 4 * lk, ncols, acc[it]);
-        }
+        tile_out<VEC>(T, acc, oz, ncols, ny, yt, (int64_t)blockIdx.x * 256);
     }
 }
 

@@ -108,7 +108,8 @@ static inline cd reref(const cd e_hat, double lns, cd kz, double z0) {
     return e_hat * cd(mag * std::cos(ph.imag()), mag * std::sin(ph.imag()));
 }
 
-static const int MY_MAX = 64;  // largest ky count of the GEMM form (K = 2*my <= 128)
+static const int MY_MAX = 64;   // largest ky count of the GEMM form (K = 2*my <= 128)
+static const int MX_MAX = 128;  // largest kx count (LDS staging of the X contraction)
 
 static void plan_region(RegionPlan& R, const scw_planewave64* W, const std::vector<double>& lns,
                         const std::vector<cd>& ehat, std::vector<float>& ellA, std::vector<int32_t>& ellC,
@@ -157,7 +158,7 @@ static void plan_region(RegionPlan& R, const scw_planewave64* W, const std::vect
     if (direct_keys) slot_of.assign((size_t)keyspace, -1);
     for (int64_t i = 0; i < n; ++i) {
         const int64_t g = R.wbegin + i;
-        const int64_t cell = (int64_t)ia[i] * R.my + ib[i];
+        const int64_t cell = (int64_t)ib[i] * R.mx + ia[i];  // b-major: C_z[:, b, :] is contiguous
         const int64_t key = cell * R.mc + ic[i];
         int64_t t;
         if (direct_keys) {
@@ -195,6 +196,10 @@ static void plan_region(RegionPlan& R, const scw_planewave64* W, const std::vect
 
     if (R.my > MY_MAX) {
         R.why_not = "more than 64 distinct ky values";
+        return;
+    }
+    if (R.mx > MX_MAX) {
+        R.why_not = "more than 128 distinct kx values";
         return;
     }
     const int64_t slots = (int64_t)width * cells;
