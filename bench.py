@@ -59,16 +59,39 @@ def time_graph(ex, out, stream, steps, warmup):
     return e0.elapsed_ms(e1) / steps, wall * 1e3 / steps
 
 
-def time_stage(ex, stage, out, stream, reps):
+def time_stages(ex, out, stream, reps):
+    """Per-stage device time inside full runs: events between the stages of
+    `reps` eager runs (coefficients, X contraction, MFMA GEMM), so every stage
+    sees the caches the previous stages left, as in a real step."""
     from scwatter_amd import hip
-    e0, e1 = hip.Event(), hip.Event()
-    ex.run_stage(stage, out.ptr, stream.handle)
-    e0.record(stream)
-    for _ in range(reps):
-        ex.run_stage(stage, out.ptr, stream.handle)
-    e1.record(stream)
-    e1.synchronize()
-    return e0.elapsed_ms(e1) / reps
+    ev = [hip.Event() for _ in range(4)]
+    tot = [0.0, 0.0, 0.0]
+    for r in range(reps + 1):
+        ev[0].record(stream)
+        for st in (1, 2, 3):
+            ex.run_stage(st, out.ptr, stream.handle)
+            ev[st].record(stream)
+        ev[3].synchronize()
+        if r:  # the first run is a warm-up
+            for st in range(3):
+                tot[st] += ev[st].elapsed_ms(ev[st + 1])
+    return [t / reps for t in tot]
+
+
+def profiled_traffic(kernel_prefix):
+    """HBM bytes per dispatch of a kernel from the committed rocprofv3 summary
+    (FETCH_SIZE / WRITE_SIZE passes scaled against a plain copy), or None."""
+    path = os.path.join(ROOT, "profiles", "tile_sample_c2_kernels.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 6 and cells[0].strip("`").startswith(kernel_prefix):
+            try:
+                return float(cells[5]) * 1e6
+            except ValueError:
+                return None
+    return None
 
 
 def roofline(flops, nbytes, ms, peak_flops):
@@ -93,14 +116,26 @@ def main():
     ap.add_argument("--sample-points", type=int, default=192)
     ap.add_argument("--cpu-rows", type=int, default=1)
     ap.add_argument("--dtypes", default="fp32,bf16x3,bf16,fp16")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 5 device-to-device copies of 512 MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the evaluation shards by z-slab with no exchange; see DESIGN.md).", file=sys.stderr)
         sys.exit(2)
 
-    import oracle
     from scwatter_amd import Exec, Plan, configs, hip
+
+    if args.copy_calibration:
+        n = 512 << 20
+        a, b = hip.DeviceArray((n,), np.uint8), hip.DeviceArray((n,), np.uint8)
+        for _ in range(5):
+            hip._ok(hip.hip().hipMemcpy(b.ptr, a.ptr, n, hip.D2D), "hipMemcpy D2D")
+        hip.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n, "copies": 5}), flush=True)
+        return
+
+    import oracle
 
     t0 = time.perf_counter()
     cw, (xs, ys, zs) = configs.config2()
@@ -132,10 +167,10 @@ def main():
         ms, wall_ms = time_graph(ex, out, stream, args.steps, args.warmup)
         E = out.to_host()
         stages = {}
+        sms = time_stages(ex, out, stream, max(3, args.steps // 2))
         for st, name in ((1, "coef"), (2, "xcontract"), (3, "main")):
-            sms = time_stage(ex, st, out, stream, max(3, args.steps // 2))
             fl, by = ex.stage_info(st)
-            stages[name] = {"ms": round(sms, 4), "flops": fl, "bytes": by}
+            stages[name] = {"ms": round(sms[st - 1], 4), "flops": fl, "bytes": by}
         ex.run(out.ptr, stream.handle)  # leave a complete result in `out`
         stream.synchronize()
         forms[f"gemm_{dt}"] = {"ms": ms, "wall_ms": wall_ms, "rate": info["units"] / (ms * 1e-3), "info": info,
@@ -198,7 +233,11 @@ def main():
     S = B["stages"][dom]
     peak = PEAK["fp32"] if dt == "fp32" or dom != "main" else PEAK["bf16"]
     roof = roofline(S["flops"], S["bytes"], S["ms"], peak)
-    roof["kernel"] = {"main": "main_f32/main_16 (MFMA)", "coef": "coef_kernel", "xcontract": "u_kernel"}[dom]
+    kname = {"coef": "scw::coef_kernel", "xcontract": "scw::u_kernel",
+             "main": "scw::main_f32" if dt == "fp32" else
+             "scw::main_16<64, true, %d>" % {"bf16": 0, "bf16x3": 1, "fp16": 2}[dt]}[dom]
+    roof["kernel"] = kname
+    roof["traffic"] = profiled_traffic(kname)
     D = forms["direct_fp32"]
     roof_direct = {"bound": "valu", "achieved": round(D["info"]["flops"] / (D["ms"] * 1e-3) / 1e12, 2),
                    "peak": PEAK["fp32"], "unit": "TFLOP/s",

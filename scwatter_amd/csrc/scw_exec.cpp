@@ -7,6 +7,7 @@
 // kernels on the caller's stream — so a run can be captured in a hipGraph.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 
@@ -235,10 +236,12 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const dou
 
     // Coefficient work items: runs of <= 8 consecutive z in one region.
     ex->items.clear();
+    std::vector<std::pair<int, int>> runs;  // (first z, first item) of each run
     for (int z = 0; z < nz;) {
         const int r = ex->zregion[z];
         int e = z + 1;
         while (e < nz && e - z < 8 && ex->zregion[e] == r) ++e;
+        runs.push_back({z, (int)ex->items.size()});
         const RegionDev& d = rd[r];
         if (d.cells > 0 && d.width > 0) {
             const int per = (d.width + d.nslots - 1) / d.nslots;
@@ -258,6 +261,45 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const dou
             b_coef += 24.0 * (double)d.cells * d.width + 8.0 * d.mc * (e - z);  // ELL amplitudes once per run, Z rows
         }
         z = e;
+    }
+    // Z groups: contiguous runs of about nz / ng slices each.
+    {
+        const int nruns = (int)runs.size();
+        // Forking several groups shrinks every launch; measured on the tile
+        // sample (21 runs, 128 z) four groups were slower than one
+        // (0.214 vs 0.197 ms), so groups are only used for grids whose
+        // single-group launches already fill the GPU several times over.
+        const int64_t waves_main = (int64_t)((6LL * nx + 255) / 256) * nz * 4;
+        int ng = (nruns >= 2 && waves_main >= 64 * 1024 && (int64_t)ex->items.size() >= 16384)
+                     ? std::min(2, nruns) : 1;
+        if (const char* env = std::getenv("SCW_ZGROUPS")) {  // override, used by the tests
+            const int v = std::atoi(env);
+            if (v >= 1) ng = std::min(v, std::max(1, nruns));
+        }
+        const int target = (nz + ng - 1) / ng;
+        ex->groups.clear();
+        ZGroup g{0, 0, 0, 0};
+        for (int k = 0; k < nruns; ++k) {
+            const int zend = k + 1 < nruns ? runs[k + 1].first : nz;
+            const int iend = k + 1 < nruns ? runs[k + 1].second : (int)ex->items.size();
+            g.z1 = zend;
+            g.i1 = iend;
+            if (g.z1 - g.z0 >= target || k + 1 == nruns) {
+                ex->groups.push_back(g);
+                g = ZGroup{zend, zend, iend, iend};
+            }
+        }
+        if (ex->groups.size() > 1) {
+            SCW_HIP(hipEventCreateWithFlags(&ex->ev_fork, hipEventDisableTiming));
+            for (size_t i = 0; i < ex->groups.size(); ++i) {
+                hipStream_t bs = nullptr;
+                SCW_HIP(hipStreamCreateWithFlags(&bs, hipStreamNonBlocking));
+                ex->bstreams.push_back(bs);
+                hipEvent_t ev = nullptr;
+                SCW_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+                ex->ev_join.push_back(ev);
+            }
+        }
     }
     ex->stage_flops[1] = f_coef;
     ex->stage_flops[2] = f_u;
@@ -303,6 +345,9 @@ static void exec_free(scw_exec* ex) {
     if (ex->graph_exec) (void)hipGraphExecDestroy(ex->graph_exec);
     if (ex->graph) (void)hipGraphDestroy(ex->graph);
     if (ex->graph_stream) (void)hipStreamDestroy(ex->graph_stream);
+    for (hipStream_t bs : ex->bstreams) (void)hipStreamDestroy(bs);
+    for (hipEvent_t ev : ex->ev_join) (void)hipEventDestroy(ev);
+    if (ex->ev_fork) (void)hipEventDestroy(ex->ev_fork);
     for (DevBuf* b : {&ex->d_dx, &ex->d_dy, &ex->d_z, &ex->d_zslab, &ex->d_items, &ex->d_regions, &ex->d_zmeta,
                       &ex->d_Z, &ex->d_X, &ex->d_C, &ex->d_U, &ex->d_Yfrag, &ex->d_out})
         b->release();

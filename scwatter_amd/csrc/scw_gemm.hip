@@ -151,10 +151,10 @@ constexpr int MX_MAX = 128;
 
 __global__ __launch_bounds__(256) void u_kernel(const ZMeta* __restrict__ zmeta, const RegionDev* __restrict__ regions,
                                                 const float2* __restrict__ C, int64_t c_elems,
-                                                const float2* __restrict__ X, int nx, float2* __restrict__ U) {
+                                                const float2* __restrict__ X, int nx, float2* __restrict__ U, int zoff) {
     __shared__ float2 sc[MX_MAX * 3];
     __shared__ float2 part[256];
-    const int z = blockIdx.z, b = blockIdx.y;
+    const int z = blockIdx.z + zoff, b = blockIdx.y;
     const ZMeta zm = zmeta[z];
     if (b >= zm.my) return;
     const RegionDev R = regions[zm.region];
@@ -253,8 +253,8 @@ __device__ inline void tile_out(float (*T)[TROW], const floatx4 (&acc)[4], float
 template <int KMAX, bool VEC>
 __global__ __launch_bounds__(256) void main_f32(const ZMeta* __restrict__ zmeta, const float2* __restrict__ U,
                                                 const float* __restrict__ Yfrag, int nx, int ny, int ytiles_per_split,
-                                                float* __restrict__ out) {
-    const int z = blockIdx.y;
+                                                float* __restrict__ out, int zoff) {
+    const int z = blockIdx.y + zoff;
     const ZMeta zm = zmeta[z];
     const int K4 = zm.kp >> 2;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -334,11 +334,11 @@ __device__ inline floatx4 mfma16(const typename Op16<MODE>::vec a, const typenam
 template <int KMAX, bool VEC, int MODE>
 __global__ __launch_bounds__(256) void main_16(const ZMeta* __restrict__ zmeta, const float2* __restrict__ U,
                                                const uint16_t* __restrict__ Yfrag, int64_t ylo_delta, int nx,
-                                               int ny, int ytiles_per_split, float* __restrict__ out) {
+                                               int ny, int ytiles_per_split, float* __restrict__ out, int zoff) {
     typedef typename Op16<MODE>::vec vec;
     typedef typename Op16<MODE>::elem elem;
     constexpr int S = KMAX / 32;
-    const int z = blockIdx.y;
+    const int z = blockIdx.y + zoff;
     const ZMeta zm = zmeta[z];
     const int K32 = zm.kp >> 5;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -398,53 +398,57 @@ __global__ __launch_bounds__(256) void main_16(const ZMeta* __restrict__ zmeta, 
 }
 
 template <int KMAX, bool VEC>
-static void launch_main(const scw_exec* ex, dim3 grid, hipStream_t st, int ytps, float* out) {
+static void launch_main(const scw_exec* ex, dim3 grid, hipStream_t st, int ytps, float* out, int zoff) {
     const ZMeta* zm = ex->d_zmeta.as<ZMeta>();
     const float2* U = ex->d_U.as<float2>();
     const int nx = (int)ex->nx, ny = (int)ex->ny;
     switch (ex->dtype) {
         case SCW_DTYPE_FP32:
             hipLaunchKernelGGL((main_f32<KMAX, VEC>), grid, dim3(256), 0, st, zm, U, ex->d_Yfrag.as<float>(), nx,
-                               ny, ytps, out);
+                               ny, ytps, out, zoff);
             break;
         case SCW_DTYPE_BF16:
             hipLaunchKernelGGL((main_16<KMAX, VEC, 0>), grid, dim3(256), 0, st, zm, U, ex->d_Yfrag.as<uint16_t>(),
-                               ex->ylo_delta, nx, ny, ytps, out);
+                               ex->ylo_delta, nx, ny, ytps, out, zoff);
             break;
         case SCW_DTYPE_BF16X3:
             hipLaunchKernelGGL((main_16<KMAX, VEC, 1>), grid, dim3(256), 0, st, zm, U, ex->d_Yfrag.as<uint16_t>(),
-                               ex->ylo_delta, nx, ny, ytps, out);
+                               ex->ylo_delta, nx, ny, ytps, out, zoff);
             break;
         default:
             hipLaunchKernelGGL((main_16<KMAX, VEC, 2>), grid, dim3(256), 0, st, zm, U, ex->d_Yfrag.as<uint16_t>(),
-                               ex->ylo_delta, nx, ny, ytps, out);
+                               ex->ylo_delta, nx, ny, ytps, out, zoff);
             break;
     }
 }
 
-int launch_gemm(const scw_exec* ex, float* out, hipStream_t st, int stage) {
-    const int nitems = (int)ex->items.size();
+// The three stages of one z group (z in [z0, z1), coefficient items [i0, i1)).
+static int launch_group(const scw_exec* ex, const ZGroup& g, float* out, hipStream_t st, int stage) {
+    const int nitems = g.i1 - g.i0;
+    const int nzg = g.z1 - g.z0;
+    if (nzg <= 0) return SCW_OK;
     if (nitems > 0 && (stage == 0 || stage == 1)) {
         hipLaunchKernelGGL(coef_kernel, dim3((unsigned)((nitems + 3) / 4)), dim3(256), 0, st,
-                           ex->d_items.as<CoefItem>(), nitems, ex->d_regions.as<RegionDev>(), ex->d_zmeta.as<ZMeta>(),
-                           ex->plan->d_ellA.as<float2>(), ex->plan->d_ellC.as<int32_t>(), ex->d_Z.as<float2>(),
-                           ex->c_elems, ex->d_C.as<float2>());
+                           ex->d_items.as<CoefItem>() + g.i0, nitems, ex->d_regions.as<RegionDev>(),
+                           ex->d_zmeta.as<ZMeta>(), ex->plan->d_ellA.as<float2>(), ex->plan->d_ellC.as<int32_t>(),
+                           ex->d_Z.as<float2>(), ex->c_elems, ex->d_C.as<float2>());
         SCW_HIP(hipGetLastError());
     }
     if (ex->my_max > 0 && (stage == 0 || stage == 2)) {
-        hipLaunchKernelGGL(u_kernel, dim3((unsigned)((ex->nx + 255) / 256), (unsigned)ex->my_max, (unsigned)ex->nz),
+        hipLaunchKernelGGL(u_kernel, dim3((unsigned)((ex->nx + 255) / 256), (unsigned)ex->my_max, (unsigned)nzg),
                            dim3(256), 0, st, ex->d_zmeta.as<ZMeta>(), ex->d_regions.as<RegionDev>(),
-                           ex->d_C.as<float2>(), ex->c_elems, ex->d_X.as<float2>(), (int)ex->nx, ex->d_U.as<float2>());
+                           ex->d_C.as<float2>(), ex->c_elems, ex->d_X.as<float2>(), (int)ex->nx, ex->d_U.as<float2>(),
+                           g.z0);
         SCW_HIP(hipGetLastError());
     }
     if (stage != 0 && stage != 3) return SCW_OK;
     const int64_t ncols = 6 * ex->nx;
-    dim3 grid((unsigned)((ncols + 255) / 256), (unsigned)ex->nz, (unsigned)ex->ysplit);
+    dim3 grid((unsigned)((ncols + 255) / 256), (unsigned)nzg, (unsigned)ex->ysplit);
     const int ytps = ex->ytiles_per_split;
     const bool vec = (ncols % 4) == 0;
 #define SCW_MAIN(K)                                                        \
-    if (vec) launch_main<K, true>(ex, grid, st, ytps, out);                \
-    else launch_main<K, false>(ex, grid, st, ytps, out);
+    if (vec) launch_main<K, true>(ex, grid, st, ytps, out, g.z0);          \
+    else launch_main<K, false>(ex, grid, st, ytps, out, g.z0);
     if (ex->dtype == SCW_DTYPE_FP32) {
         switch (ex->kmax) {
             case 16: SCW_MAIN(16) break;
@@ -462,6 +466,32 @@ int launch_gemm(const scw_exec* ex, float* out, hipStream_t st, int stage) {
     }
 #undef SCW_MAIN
     SCW_HIP(hipGetLastError());
+    return SCW_OK;
+}
+
+// A full run forks the z groups onto the exec's branch streams so that one
+// group's read-bound coefficient stage overlaps another group's write-bound
+// MFMA stage, and joins them back onto the caller's stream.  Captured into a
+// hipGraph this becomes parallel branches.  Single stages (profiling) run
+// every group in order on the caller's stream.
+int launch_gemm(const scw_exec* ex, float* out, hipStream_t st, int stage) {
+    const int ng = (int)ex->groups.size();
+    if (stage != 0 || ng <= 1) {
+        for (const ZGroup& g : ex->groups) {
+            const int rc = launch_group(ex, g, out, st, stage);
+            if (rc != SCW_OK) return rc;
+        }
+        return SCW_OK;
+    }
+    SCW_HIP(hipEventRecord(ex->ev_fork, st));
+    for (int i = 0; i < ng; ++i) {
+        hipStream_t bs = ex->bstreams[i];
+        SCW_HIP(hipStreamWaitEvent(bs, ex->ev_fork, 0));
+        const int rc = launch_group(ex, ex->groups[i], out, bs, 0);
+        if (rc != SCW_OK) return rc;
+        SCW_HIP(hipEventRecord(ex->ev_join[i], bs));
+    }
+    for (int i = 0; i < ng; ++i) SCW_HIP(hipStreamWaitEvent(st, ex->ev_join[i], 0));
     return SCW_OK;
 }
 

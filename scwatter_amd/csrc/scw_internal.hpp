@@ -117,6 +117,11 @@ struct RegionDev {      // per-region data the GEMM-form kernels read (device)
     int32_t mx, my, nslots, pad;
 };
 
+// A contiguous range of grid z and the coefficient items that serve it.
+struct ZGroup {
+    int z0, z1, i0, i1;
+};
+
 // Direct form helpers (scw_direct.hip).
 size_t direct_region_bytes();
 void direct_region_fill(void* dst, int64_t begin, int64_t end, int real_transverse);
@@ -149,6 +154,10 @@ struct scw_exec {
     int64_t ylo_delta = 0;    // element offset of the lo Y fragments (bf16 x3)
     int ysplit = 1;           // grid.z of the main GEMM
     int ytiles_per_split = 1;
+    std::vector<scw::ZGroup> groups;        // z groups of the GEMM form (>= 1)
+    std::vector<hipStream_t> bstreams;      // one branch stream per group when groups > 1
+    hipEvent_t ev_fork = nullptr;
+    std::vector<hipEvent_t> ev_join;
     scw::DevBuf d_out;        // staging for scw_exec_run_host
     // graph
     hipGraph_t graph = nullptr;

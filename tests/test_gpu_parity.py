@@ -239,3 +239,21 @@ def test_errors_are_reported():
         Plan(WaveTables(W, np.zeros(0), np.array([0], np.int32), np.array([n + 1], np.int32)))
     with pytest.raises(ScwError, match="ARG"):
         Plan(WaveTables(W, np.zeros(0), np.array([0], np.int32), np.array([n], np.int32)), device=99)
+
+
+def test_forked_z_groups_match_single_group(tile_small, monkeypatch):
+    """The fork/join z-group path (branch streams, also inside a captured graph) is bit-identical."""
+    from scwatter_amd import hip
+    plan, g, _ = tile_small
+    one = Exec(plan, *g, form="gemm", dtype="bf16x3").run_host()
+    monkeypatch.setenv("SCW_ZGROUPS", "3")
+    ex = Exec(plan, *g, form="gemm", dtype="bf16x3")
+    assert np.array_equal(ex.run_host(), one)
+    s = hip.Stream()
+    buf = hip.DeviceArray(ex.shape, np.complex64)
+    ex.graph_capture(buf.ptr, s.handle)
+    for _ in range(2):
+        ex.graph_launch(s.handle)
+        s.synchronize()
+        assert np.array_equal(buf.to_host(), one)
+    s.destroy()
