@@ -46,6 +46,14 @@ static inline uint16_t f16_rne(float f) {
 
 static inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
+// Launch-shape defaults, overridable through the environment for tuning sweeps.
+static int64_t tunable(const char* name, int64_t dflt) {
+    const char* v = std::getenv(name);
+    if (!v) return dflt;
+    const long long x = std::atoll(v);
+    return x > 0 ? (int64_t)x : dflt;
+}
+
 static int prep_direct(scw_exec* ex, const double* xs, const double* ys, const double* zs) {
     const scw_plan* P = ex->plan;
     const double xmin = *std::min_element(xs, xs + ex->nx), xmax = *std::max_element(xs, xs + ex->nx);
@@ -93,7 +101,7 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const dou
     const int L = P->layers;
     const int nx = (int)ex->nx, ny = (int)ex->ny, nz = (int)ex->nz;
     const bool is16 = ex->dtype != SCW_DTYPE_FP32;
-    const int kq = is16 ? 32 : 4;  // MFMA K granule
+    const int kq = is16 ? 32 : 16;  // K granule: one MFMA (16-bit) or one 16-byte Y load of four k-steps (FP32)
     std::vector<char> used(L + 1, 0);
     for (int z = 0; z < nz; ++z) used[ex->zregion[z]] = 1;
 
@@ -146,10 +154,13 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const dou
         auto yval = [&](int y, int k) { return (y < ny && k < kp) ? Yp[(size_t)y * kp + k] : 0.0; };
         if (!is16) {
             yfrag_off[r] = (int64_t)yf32.size();
-            const int K4 = kp / 4;
+            // [yt][q][lane][e]: lane l holds k = 4 (4q + e) + (l >> 4) of row 16 yt + (l & 15)
+            const int K16 = kp / 16;
             for (int yt = 0; yt < ytiles; ++yt)
-                for (int s = 0; s < K4; ++s)
-                    for (int l = 0; l < 64; ++l) yf32.push_back((float)yval(16 * yt + (l & 15), 4 * s + (l >> 4)));
+                for (int q = 0; q < K16; ++q)
+                    for (int l = 0; l < 64; ++l)
+                        for (int e = 0; e < 4; ++e)
+                            yf32.push_back((float)yval(16 * yt + (l & 15), 4 * (4 * q + e) + (l >> 4)));
         } else {
             yfrag_off[r] = (int64_t)yhi.size();
             const int K32 = kp / 32;
@@ -187,7 +198,8 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const dou
             base += (rd[r].cells + 63) / 64;
             z = e;
         }
-        const int want = base > 0 ? (int)std::min<int64_t>(64, (4096 + base - 1) / base) : 1;
+        const int64_t target = tunable("SCW_COEF_ITEMS", 4096);
+        const int want = base > 0 ? (int)std::min<int64_t>(64, (target + base - 1) / base) : 1;
         for (int r = 0; r <= L; ++r)
             if (used[r]) {
                 rd[r].nslots = std::max(1, std::min(want, rd[r].width / 16));
@@ -315,7 +327,8 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const dou
     // Each wave keeps its B-operand fragment (from U_z) in registers for all of
     // its y tiles, so split y across workgroups only as far as needed to give
     // every CU a couple of workgroups.
-    int ysplit = (int)std::min<int64_t>(ytiles, std::max<int64_t>(1, (512 + (int64_t)gx * nz - 1) / ((int64_t)gx * nz)));
+    const int64_t mtarget = tunable("SCW_MAIN_BLOCKS", 512);
+    int ysplit = (int)std::min<int64_t>(ytiles, std::max<int64_t>(1, (mtarget + (int64_t)gx * nz - 1) / ((int64_t)gx * nz)));
     ex->ytiles_per_split = (ytiles + ysplit - 1) / ysplit;
     ex->ysplit = (ytiles + ex->ytiles_per_split - 1) / ex->ytiles_per_split;
 

@@ -214,8 +214,10 @@ __device__ inline float bp_value(const float* __restrict__ Ub, int64_t ncols, in
 // Epilogue of one 16-row y tile of a workgroup (256 output columns): every
 // wave parks its 16 x 64 accumulator tile in LDS, then each store
 // instruction writes one full 1 KB row segment (lane l: columns 4l..4l+3).
-// The field is written once and not read back, so the stores are
-// non-temporal.
+// T is double-buffered by tile parity, so one barrier per tile suffices: a
+// wave can only pass the barrier of tile t once every wave has finished
+// reading the buffer of tile t-1, which tile t+1 then reuses.  The field is
+// written once and not read back, so the stores are non-temporal.
 constexpr int TROW = 256 + 4;
 
 template <bool VEC>
@@ -242,27 +244,29 @@ __device__ inline void tile_out(float (*T)[TROW], const floatx4 (&acc)[4], float
             }
         }
     }
-    __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
 // Main GEMM, FP32 operands: v_mfma_f32_16x16x4_f32.
 // Workgroup = 4 waves, each owns 64 output columns (n) of one z-slice and
-// walks y in tiles of 16; its B-operand fragment (from U) stays in registers.
+// walks y in tiles of 16; its A-operand fragment (from U) stays in registers.
+// The B-operand (Y) fragment of a tile is four k-steps per 16-byte load,
+// prefetched one tile ahead.
 // ---------------------------------------------------------------------------
 template <int KMAX, bool VEC>
 __global__ __launch_bounds__(256) void main_f32(const ZMeta* __restrict__ zmeta, const float2* __restrict__ U,
                                                 const float* __restrict__ Yfrag, int nx, int ny, int ytiles_per_split,
                                                 float* __restrict__ out, int zoff) {
+    constexpr int Q = KMAX / 16;  // 16-byte Y loads per tile
     const int z = blockIdx.y + zoff;
     const ZMeta zm = zmeta[z];
-    const int K4 = zm.kp >> 2;
+    const int K16 = zm.kp >> 4;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t ncols = 6LL * nx;
     const int64_t n0 = (int64_t)blockIdx.x * 256 + wv * 64;  // waves past ncols compute zeros
     const int li = lane & 15, lk = lane >> 4;
     const float* Ub = reinterpret_cast<const float*>(U + zm.u_off);
-    __shared__ float T[16][TROW];
+    __shared__ float T[2][16][TROW];
 
     float uf[4][KMAX / 4];
 #pragma unroll
@@ -271,30 +275,38 @@ __global__ __launch_bounds__(256) void main_f32(const ZMeta* __restrict__ zmeta,
 #pragma unroll
         for (int s = 0; s < KMAX / 4; ++s) {
             const int k = 4 * s + lk;
-            uf[it][s] = (s < K4 && (k >> 1) < zm.my && n < ncols) ? bp_value(Ub, ncols, k, n) : 0.0f;
+            uf[it][s] = ((s >> 2) < K16 && (k >> 1) < zm.my && n < ncols) ? bp_value(Ub, ncols, k, n) : 0.0f;
         }
     }
 
     const int ytiles = (ny + 15) >> 4;
     const int yt0 = blockIdx.z * ytiles_per_split;
     const int yt1 = min(ytiles, yt0 + ytiles_per_split);
-    const float* Yf = Yfrag + zm.yfrag_off;
+    const floatx4* Yf = reinterpret_cast<const floatx4*>(Yfrag + zm.yfrag_off) + lane;
     float* oz = out + (int64_t)z * ny * ncols;
+    floatx4 ycur[Q], ynext[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) ycur[q] = (q < K16 && yt0 < yt1) ? Yf[((int64_t)yt0 * K16 + q) * 64] : floatx4{0, 0, 0, 0};
     for (int yt = yt0; yt < yt1; ++yt) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+            ynext[q] = (q < K16 && yt + 1 < yt1) ? Yf[((int64_t)(yt + 1) * K16 + q) * 64] : floatx4{0, 0, 0, 0};
         floatx4 acc[4];
 #pragma unroll
         for (int it = 0; it < 4; ++it) acc[it] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-        const float* yrow = Yf + (int64_t)yt * K4 * 64 + lane;
 #pragma unroll
-        for (int s = 0; s < KMAX / 4; ++s) {
-            if (s < K4) {
-                const float yv = yrow[s * 64];
+        for (int q = 0; q < Q; ++q) {
+            if (q < K16) {
 #pragma unroll
-                for (int it = 0; it < 4; ++it)
-                    acc[it] = __builtin_amdgcn_mfma_f32_16x16x4f32(uf[it][s], yv, acc[it], 0, 0, 0);
+                for (int e = 0; e < 4; ++e)
+#pragma unroll
+                    for (int it = 0; it < 4; ++it)
+                        acc[it] = __builtin_amdgcn_mfma_f32_16x16x4f32(uf[it][4 * q + e], ycur[q][e], acc[it], 0, 0, 0);
             }
         }
-        tile_out<VEC>(T, acc, oz, ncols, ny, yt, (int64_t)blockIdx.x * 256);
+        tile_out<VEC>(T[yt & 1], acc, oz, ncols, ny, yt, (int64_t)blockIdx.x * 256);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) ycur[q] = ynext[q];
     }
 }
 
@@ -346,7 +358,7 @@ __global__ __launch_bounds__(256) void main_16(const ZMeta* __restrict__ zmeta, 
     const int64_t n0 = (int64_t)blockIdx.x * 256 + wv * 64;  // waves past ncols compute zeros
     const int li = lane & 15, lk = lane >> 4;
     const float* Ub = reinterpret_cast<const float*>(U + zm.u_off);
-    __shared__ float T[16][TROW];
+    __shared__ float T[2][16][TROW];
 
     vec ahi[4][S], alo[4][S];
 #pragma unroll
@@ -368,32 +380,44 @@ __global__ __launch_bounds__(256) void main_16(const ZMeta* __restrict__ zmeta, 
     const int ytiles = (ny + 15) >> 4;
     const int yt0 = blockIdx.z * ytiles_per_split;
     const int yt1 = min(ytiles, yt0 + ytiles_per_split);
-    const uint16_t* Yf = Yfrag + zm.yfrag_off;
+    const uint16_t* Yf = Yfrag + zm.yfrag_off + (int64_t)lane * 8;
     float* oz = out + (int64_t)z * ny * ncols;
+    // Y fragments of the current tile and the next one (prefetched).
+    vec yh[S], yl[S], nh[S], nl[S];
+    auto yload = [&](int yt, vec (&h)[S], vec (&l)[S]) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const bool ok = s < K32 && yt < yt1;
+            const uint16_t* p = Yf + ((int64_t)yt * K32 + s) * 512;
+            h[s] = ok ? *reinterpret_cast<const vec*>(p) : vec{};
+            if constexpr (MODE == 1) l[s] = ok ? *reinterpret_cast<const vec*>(p + ylo_delta) : vec{};
+        }
+    };
+    yload(yt0, yh, yl);
     for (int yt = yt0; yt < yt1; ++yt) {
+        yload(yt + 1, nh, nl);
         floatx4 acc[4];
 #pragma unroll
         for (int it = 0; it < 4; ++it) acc[it] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-        const uint16_t* yrow = Yf + ((int64_t)yt * K32 * 64 + lane) * 8;
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             if (s < K32) {
-                const vec yh = *reinterpret_cast<const vec*>(yrow + (int64_t)s * 512);
                 if constexpr (MODE == 1) {
-                    const vec yl = *reinterpret_cast<const vec*>(yrow + (int64_t)s * 512 + ylo_delta);
 #pragma unroll
-                    for (int it = 0; it < 4; ++it) {
-                        acc[it] = mfma16<MODE>(alo[it][s], yh, acc[it]);
-                        acc[it] = mfma16<MODE>(ahi[it][s], yl, acc[it]);
-                        acc[it] = mfma16<MODE>(ahi[it][s], yh, acc[it]);
-                    }
-                } else {
+                    for (int it = 0; it < 4; ++it) acc[it] = mfma16<MODE>(alo[it][s], yh[s], acc[it]);
 #pragma unroll
-                    for (int it = 0; it < 4; ++it) acc[it] = mfma16<MODE>(ahi[it][s], yh, acc[it]);
+                    for (int it = 0; it < 4; ++it) acc[it] = mfma16<MODE>(ahi[it][s], yl[s], acc[it]);
                 }
+#pragma unroll
+                for (int it = 0; it < 4; ++it) acc[it] = mfma16<MODE>(ahi[it][s], yh[s], acc[it]);
             }
         }
-        tile_out<VEC>(T, acc, oz, ncols, ny, yt, (int64_t)blockIdx.x * 256);
+        tile_out<VEC>(T[yt & 1], acc, oz, ncols, ny, yt, (int64_t)blockIdx.x * 256);
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            yh[s] = nh[s];
+            if constexpr (MODE == 1) yl[s] = nl[s];
+        }
     }
 }
 
@@ -402,23 +426,22 @@ static void launch_main(const scw_exec* ex, dim3 grid, hipStream_t st, int ytps,
     const ZMeta* zm = ex->d_zmeta.as<ZMeta>();
     const float2* U = ex->d_U.as<float2>();
     const int nx = (int)ex->nx, ny = (int)ex->ny;
-    switch (ex->dtype) {
-        case SCW_DTYPE_FP32:
-            hipLaunchKernelGGL((main_f32<KMAX, VEC>), grid, dim3(256), 0, st, zm, U, ex->d_Yfrag.as<float>(), nx,
-                               ny, ytps, out, zoff);
-            break;
-        case SCW_DTYPE_BF16:
-            hipLaunchKernelGGL((main_16<KMAX, VEC, 0>), grid, dim3(256), 0, st, zm, U, ex->d_Yfrag.as<uint16_t>(),
-                               ex->ylo_delta, nx, ny, ytps, out, zoff);
-            break;
-        case SCW_DTYPE_BF16X3:
-            hipLaunchKernelGGL((main_16<KMAX, VEC, 1>), grid, dim3(256), 0, st, zm, U, ex->d_Yfrag.as<uint16_t>(),
-                               ex->ylo_delta, nx, ny, ytps, out, zoff);
-            break;
-        default:
-            hipLaunchKernelGGL((main_16<KMAX, VEC, 2>), grid, dim3(256), 0, st, zm, U, ex->d_Yfrag.as<uint16_t>(),
-                               ex->ylo_delta, nx, ny, ytps, out, zoff);
-            break;
+    if (ex->dtype == SCW_DTYPE_FP32) {
+        hipLaunchKernelGGL((main_f32<KMAX, VEC>), grid, dim3(256), 0, st, zm, U, ex->d_Yfrag.as<float>(), nx, ny,
+                           ytps, out, zoff);
+        return;
+    }
+    if constexpr (KMAX >= 32) {
+        const uint16_t* Y = ex->d_Yfrag.as<uint16_t>();
+        if (ex->dtype == SCW_DTYPE_BF16)
+            hipLaunchKernelGGL((main_16<KMAX, VEC, 0>), grid, dim3(256), 0, st, zm, U, Y, ex->ylo_delta, nx, ny,
+                               ytps, out, zoff);
+        else if (ex->dtype == SCW_DTYPE_BF16X3)
+            hipLaunchKernelGGL((main_16<KMAX, VEC, 1>), grid, dim3(256), 0, st, zm, U, Y, ex->ylo_delta, nx, ny,
+                               ytps, out, zoff);
+        else
+            hipLaunchKernelGGL((main_16<KMAX, VEC, 2>), grid, dim3(256), 0, st, zm, U, Y, ex->ylo_delta, nx, ny,
+                               ytps, out, zoff);
     }
 }
 
