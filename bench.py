@@ -107,6 +107,27 @@ def roofline(flops, nbytes, ms, peak_flops):
             "frac": round(a / peak_flops, 4), "traffic": None}
 
 
+WORKLOADS = {
+    "c2": ("sCWatter tile sample 21x21 orders, 20 sublayers, 15,559,803 waves; grid 256x256x128", "config2"),
+    "c1": ("single interface n 1.0 / 1.4+0.05i, 21x21 focused beam, NA 0.5; grid 64^3", "config1"),
+    "c3": ("single interface, 41x41 pupil x 2 polarisations (6,724 waves for z < 0, 3,362 above); grid 512x512x256 "
+           "over [-60, 60]^3 um", "config3"),
+    "c5": ("16-layer stack, 31x31 focused beam NA 0.7, lambda 0.6 um (33,635 waves); grid 1024x1024x512",
+           "config5"),
+}
+
+
+def sample_device(out, shape, pidx):
+    """Copy the field at the sample indices from device memory (24 bytes each)."""
+    from scwatter_amd import hip
+    res = np.empty((len(pidx), 3), np.complex64)
+    nz, ny, nx, _ = shape
+    for i, (iz, iy, ix) in enumerate(pidx):
+        off = ((int(iz) * ny + int(iy)) * nx + int(ix)) * 24
+        hip._ok(hip.hip().hipMemcpy(res[i].ctypes.data, out.ptr + off, 24, hip.D2H), "hipMemcpy D2H")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,8 +135,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--direct-steps", type=int, default=1)
     ap.add_argument("--sample-points", type=int, default=192)
-    ap.add_argument("--cpu-rows", type=int, default=1)
+    ap.add_argument("--cpu-units", type=float, default=2e8, help="size of the CPU-baseline sample (pts x waves)")
     ap.add_argument("--dtypes", default="fp32,bf16x3,bf16,fp16")
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device-to-device copies of 512 MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -137,11 +159,13 @@ def main():
 
     import oracle
 
+    desc, gen = WORKLOADS[args.workload]
     t0 = time.perf_counter()
-    cw, (xs, ys, zs) = configs.config2()
+    cw, (xs, ys, zs) = getattr(configs, gen)()
     tables = cw.unpack()
     del cw
-    log(f"[bench] structure: {tables.W.shape[0]} waves, {tables.layers} boundaries ({time.perf_counter()-t0:.1f} s)")
+    log(f"[bench] {args.workload}: {tables.W.shape[0]} waves, {tables.layers} boundaries, grid "
+        f"{xs.size}x{ys.size}x{zs.size} ({time.perf_counter()-t0:.1f} s)")
     t0 = time.perf_counter()
     plan = Plan(tables, device=0)
     log(f"[bench] plan: {plan.device_bytes()/1e9:.2f} GB on device ({time.perf_counter()-t0:.1f} s)")
@@ -154,9 +178,6 @@ def main():
     pidx = np.stack([rng.integers(0, n, args.sample_points) for n in (zs.size, ys.size, xs.size)], 1)
     pts = np.stack([xs[pidx[:, 2]], ys[pidx[:, 1]], zs[pidx[:, 0]]], 1)
 
-    def sample(E):
-        return E[pidx[:, 0], pidx[:, 1], pidx[:, 2]]
-
     # ---- GEMM form, every dtype ----
     forms = {}
     dtypes = [d for d in args.dtypes.split(",") if d]
@@ -165,21 +186,18 @@ def main():
         ex = Exec(plan, xs, ys, zs, form="gemm", dtype=dt)
         info = ex.info()
         ms, wall_ms = time_graph(ex, out, stream, args.steps, args.warmup)
-        E = out.to_host()
+        samp = sample_device(out, shape, pidx)
         stages = {}
         sms = time_stages(ex, out, stream, max(3, args.steps // 2))
         for st, name in ((1, "coef"), (2, "xcontract"), (3, "main")):
             fl, by = ex.stage_info(st)
             stages[name] = {"ms": round(sms[st - 1], 4), "flops": fl, "bytes": by}
-        ex.run(out.ptr, stream.handle)  # leave a complete result in `out`
-        stream.synchronize()
         forms[f"gemm_{dt}"] = {"ms": ms, "wall_ms": wall_ms, "rate": info["units"] / (ms * 1e-3), "info": info,
-                               "stages": stages, "sample": sample(E)}
+                               "stages": stages, "sample": samp}
         log(f"[bench] gemm {dt}: {ms:.3f} ms/step (wall {wall_ms:.3f}), "
             f"{info['units'] / (ms * 1e-3):.3e} pts*waves/s, stages " +
             ", ".join(f"{k} {v['ms']:.3f} ms" for k, v in stages.items()) + f" ({time.perf_counter()-t0:.1f} s)")
         ex.destroy()
-        del E
 
     # ---- direct form ----
     t0 = time.perf_counter()
@@ -196,12 +214,11 @@ def main():
     e1.record(stream)
     e1.synchronize()
     dms = e0.elapsed_ms(e1) / args.direct_steps
-    E = out.to_host()
-    forms["direct_fp32"] = {"ms": dms, "rate": info["units"] / (dms * 1e-3), "info": info, "sample": sample(E)}
+    forms["direct_fp32"] = {"ms": dms, "rate": info["units"] / (dms * 1e-3), "info": info,
+                            "sample": sample_device(out, shape, pidx)}
     log(f"[bench] direct fp32: {dms:.1f} ms/step, {info['units'] / (dms * 1e-3):.3e} pts*waves/s "
         f"({time.perf_counter()-t0:.1f} s)")
     ex.destroy()
-    del E
 
     # ---- CPU leg: FP64 oracle on the sample points (checker) + reference-semantics FP32 baseline ----
     t0 = time.perf_counter()
@@ -212,11 +229,14 @@ def main():
         f["rel_l2"] = oracle.rel_l2(f["sample"], ref)
         f["max_abs"] = oracle.max_abs(f["sample"], ref)
         log(f"[bench] {k}: rel-L2 {f['rel_l2']:.3e} max-abs {f['max_abs']:.3e}")
-    # CPU baseline: cpu_cw_evaluate_xy semantics on `cpu_rows` rows of one internal-region slice.
+    # CPU baseline: cpu_cw_evaluate_xy semantics on whole rows of the slice nearest z = 0.3 um,
+    # as many rows as make about --cpu-units units.
     iz = int(np.argmin(np.abs(zs - 0.3)))
-    xs32, ys32, zs32 = xs.astype(np.float32), ys[: args.cpu_rows].astype(np.float32), zs[iz:iz + 1].astype(np.float32)
-    r = oracle.region_of(float(zs32[0]), tables.z_layers)
-    cpu_units = float(xs.size * args.cpu_rows * (tables.end[r] - tables.begin[r]))
+    r = oracle.region_of(float(np.float32(zs[iz])), tables.z_layers.astype(np.float32))
+    w_r = int(tables.end[r] - tables.begin[r])
+    rows = int(max(1, min(ys.size, round(args.cpu_units / max(1, xs.size * w_r)))))
+    xs32, ys32, zs32 = xs.astype(np.float32), ys[:rows].astype(np.float32), zs[iz:iz + 1].astype(np.float32)
+    cpu_units = float(xs.size * rows * w_r)
     t0 = time.perf_counter()
     oracle.refcpu_grid(tables.W, tables.z_layers, tables.begin, tables.end, xs32, ys32, zs32, nthreads=1)
     cpu_s = time.perf_counter() - t0
@@ -237,7 +257,7 @@ def main():
              "main": "scw::main_f32" if dt == "fp32" else
              "scw::main_16<64, true, %d>" % {"bf16": 0, "bf16x3": 1, "fp16": 2}[dt]}[dom]
     roof["kernel"] = kname
-    roof["traffic"] = profiled_traffic(kname)
+    roof["traffic"] = profiled_traffic(kname) if args.workload == "c2" else None
     D = forms["direct_fp32"]
     roof_direct = {"bound": "valu", "achieved": round(D["info"]["flops"] / (D["ms"] * 1e-3) / 1e12, 2),
                    "peak": PEAK["fp32"], "unit": "TFLOP/s",
@@ -247,9 +267,7 @@ def main():
         "metric": METRIC, "value": B["rate"], "unit": UNIT, "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": B["ms"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": dt, "data": "synthetic",
-        "config": {"workload": "sCWatter tile sample 21x21 orders, 20 sublayers, 15,559,803 waves; "
-                               "grid 256x256x128", "units_per_step": B["info"]["units"], "form": "gemm",
-                   "gemm_dtype": dt},
+        "config": {"workload": desc, "units_per_step": B["info"]["units"], "form": "gemm", "gemm_dtype": dt},
         "rel_l2": B["rel_l2"], "max_abs": B["max_abs"], "accuracy_sample_points": len(pts),
         "gemm_over_direct": B["rate"] / D["rate"],
         "forms": {k: {"ms_per_step": round(f["ms"], 4), "rate": f["rate"], "rel_l2": f["rel_l2"],
@@ -257,9 +275,9 @@ def main():
                                                   if "stages" in f else {})} for k, f in forms.items()},
         "roofline": roof, "roofline_direct": roof_direct,
         "cpu_baseline": {"value": cpu_rate, "unit": UNIT, "cores": 1, "kind": "port",
-                         "sample": f"cpu_cw_evaluate_xy semantics (FP32, 1 thread) on {args.cpu_rows} row(s) x "
-                                   f"{xs.size} points of the z={zs[iz]:.3f} slice (region {r}, "
-                                   f"{tables.end[r] - tables.begin[r]} waves) = {cpu_units:.3e} units"},
+                         "sample": f"cpu_cw_evaluate_xy semantics (FP32, 1 thread) on {rows} row(s) x "
+                                   f"{xs.size} points of the z={zs[iz]:.3f} slice (region {r}, {w_r} waves) = "
+                                   f"{cpu_units:.3e} units"},
     }
     print(json.dumps(line), flush=True)
 
