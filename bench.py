@@ -20,6 +20,7 @@ rel-L2 figures are measured against.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--direct-steps D]
 """
 import argparse
+import collections
 import json
 import os
 import sys
@@ -114,6 +115,8 @@ WORKLOADS = {
            "over [-60, 60]^3 um", "config3"),
     "c5": ("16-layer stack, 31x31 focused beam NA 0.7, lambda 0.6 um (33,635 waves); grid 1024x1024x512",
            "config5"),
+    "c4": ("batched sweep: 64 tile-sample slabs, lambda 0.4-1.0 um, incidence 0-40 deg, orders from "
+           "{11^2, 21^2, 31^2, 41^2}; 256x256x64 per instance, one batch run", "config4"),
 }
 
 
@@ -158,6 +161,9 @@ def main():
         return
 
     import oracle
+
+    if args.workload == "c4":
+        return bench_batch(args)
 
     desc, gen = WORKLOADS[args.workload]
     t0 = time.perf_counter()
@@ -278,6 +284,105 @@ def main():
                          "sample": f"cpu_cw_evaluate_xy semantics (FP32, 1 thread) on {rows} row(s) x "
                                    f"{xs.size} points of the z={zs[iz]:.3f} slice (region {r}, {w_r} waves) = "
                                    f"{cpu_units:.3e} units"},
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_batch(args):
+    """Batched sweep (config 4): every instance in one GEMM-form run, launch shape from the autotuner."""
+    import oracle
+    from scwatter_amd import BatchExec, Plan, autotune, configs, hip
+
+    t0 = time.perf_counter()
+    params = configs.config4_instances()
+    xs, ys, zs = configs.config4_grid()
+    nzi = zs.size
+    rng = np.random.default_rng(20261203)
+    samples = [(i, int(rng.integers(nzi)), int(rng.integers(ys.size)), int(rng.integers(xs.size)))
+               for i in range(len(params)) for _ in range(3)]
+    # Each instance is planned, checked by the oracle at its 3 sample points
+    # (the CPU leg's checker), and its host tables dropped, except the
+    # largest instance's, kept for the CPU baseline.
+    plans, ref_parts, waves, big = [], [], 0, None
+    nthr = min(16, os.cpu_count() or 1)
+    t_or = 0.0
+    for i, p in enumerate(params):
+        t = configs.config4_structure(p).unpack()
+        plans.append(Plan(t, device=0))
+        waves += t.W.shape[0]
+        t2 = time.perf_counter()
+        pts = [[xs[x], ys[y], zs[z]] for j, z, y, x in samples if j == i]
+        ref_parts.append(oracle.eval_points_f64(t.W, t.z_layers, t.begin, t.end, pts, nthreads=nthr))
+        t_or += time.perf_counter() - t2
+        if big is None or t.W.shape[0] > big.W.shape[0]:
+            big = t
+        del t
+        if i % 8 == 7:
+            log(f"[bench] c4: {i + 1} instances planned ({time.perf_counter()-t0:.1f} s)")
+    ref = np.concatenate(ref_parts)
+    log(f"[bench] c4: {len(plans)} instances, {waves} waves, orders "
+        f"{sorted(collections.Counter(p['M'] for p in params).items())} ({time.perf_counter()-t0:.1f} s, "
+        f"oracle {t_or:.1f} s)")
+    stream = hip.Stream()
+    forms = {}
+    for dt in [d for d in args.dtypes.split(",") if d in ("bf16x3", "fp32", "bf16", "fp16")]:
+        t1 = time.perf_counter()
+        bx = BatchExec(plans, xs, ys, [zs] * len(plans), dtype=dt)
+        out = hip.DeviceArray(bx.shape, np.complex64)
+        best, cand = autotune.tune(bx, out.ptr, stream, reps=3)
+        info = bx.info()
+        ms, wall_ms = time_graph(bx, out, stream, args.steps, args.warmup)
+        sms = time_stages(bx, out, stream, 3)
+        stages = {}
+        for st, name in ((1, "coef"), (2, "xcontract"), (3, "main")):
+            fl, by = bx.stage_info(st)
+            stages[name] = {"ms": round(sms[st - 1], 4), "flops": fl, "bytes": by}
+        pidx = np.array([(i * nzi + z, y, x) for i, z, y, x in samples])
+        samp = sample_device(out, bx.shape, pidx)
+        forms[f"gemm_{dt}"] = {"ms": ms, "wall_ms": wall_ms, "rate": info["units"] / (ms * 1e-3), "info": info,
+                               "stages": stages, "sample": samp, "tuned": {"coef_items": best[0], "main_blocks": best[1],
+                                                                           "candidates_ms": {f"{a}/{b}": round(v, 4)
+                                                                                             for (a, b), v in cand.items()}}}
+        log(f"[bench] c4 gemm {dt}: {ms:.3f} ms/step, {info['units'] / (ms * 1e-3):.3e} pts*waves/s, tuned {best}, "
+            "stages " + ", ".join(f"{k} {v['ms']:.3f} ms" for k, v in stages.items()) +
+            f" ({time.perf_counter()-t1:.1f} s)")
+        bx.destroy()
+        out.free()
+    for k, f in forms.items():
+        f["rel_l2"] = oracle.rel_l2(f["sample"], ref)
+        f["max_abs"] = oracle.max_abs(f["sample"], ref)
+        log(f"[bench] c4 {k}: rel-L2 {f['rel_l2']:.3e} max-abs {f['max_abs']:.3e}")
+    # CPU baseline: reference-semantics FP32 port on a bounded sample of the largest instance
+    tb = big
+    w_r = int(tb.end[1] - tb.begin[1])
+    npts = int(max(1, min(xs.size, round(args.cpu_units / w_r))))
+    t1 = time.perf_counter()
+    oracle.refcpu_grid(tb.W, tb.z_layers, tb.begin, tb.end, xs[:npts].astype(np.float32),
+                       ys[:1].astype(np.float32), zs[nzi // 2:nzi // 2 + 1].astype(np.float32), nthreads=1)
+    cpu_s = time.perf_counter() - t1
+    cpu_units = float(npts * w_r)
+    ok = [k for k in forms if forms[k]["rel_l2"] <= ACCURACY_BAR]
+    best = max(ok or list(forms), key=lambda k: forms[k]["rate"])
+    B = forms[best]
+    dt = best[len("gemm_"):]
+    dom = max(B["stages"], key=lambda k: B["stages"][k]["ms"])
+    S = B["stages"][dom]
+    roof = roofline(S["flops"], S["bytes"], S["ms"], PEAK["fp32"] if dt == "fp32" else PEAK["bf16"])
+    roof["kernel"] = {"coef": "scw::modal_kernel", "xcontract": "scw::u_kernel", "main": "scw::main"}[dom]
+    line = {
+        "metric": METRIC, "value": B["rate"], "unit": UNIT, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": B["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dt,
+        "data": "synthetic",
+        "config": {"workload": WORKLOADS["c4"][0], "units_per_step": B["info"]["units"], "form": "gemm",
+                   "gemm_dtype": dt, "instances": len(plans), "waves": waves},
+        "rel_l2": B["rel_l2"], "max_abs": B["max_abs"], "accuracy_sample_points": len(samples),
+        "forms": {k: {"ms_per_step": round(f["ms"], 4), "rate": f["rate"], "rel_l2": f["rel_l2"],
+                      "max_abs": f["max_abs"], "stages_ms": {s: v["ms"] for s, v in f["stages"].items()},
+                      "tuned": f["tuned"]} for k, f in forms.items()},
+        "roofline": roof,
+        "cpu_baseline": {"value": cpu_units / cpu_s, "unit": UNIT, "cores": 1, "kind": "port",
+                         "sample": f"cpu_cw_evaluate_xy semantics (FP32, 1 thread) on {npts} points of one row of the "
+                                   f"largest instance ({w_r} waves) = {cpu_units:.3e} units"},
     }
     print(json.dumps(line), flush=True)
 

@@ -119,6 +119,19 @@ int scw_exec_info(const scw_exec* exec, int* form, int* dtype, double* units, do
  * multiply-add whatever the dtype; bytes count each table read and each
  * result written once. */
 int scw_exec_stage_info(const scw_exec* exec, int stage, double* flops, double* bytes);
+/* Batch of independent structures in one run (GEMM form): instance i is
+ * plans[i] on the grid xs x ys x zs[i] (nz[i] slices; shared x and y axes).
+ * Output: the instances' fields concatenated along z,
+ * complex64 [sum nz[i]][ny][nx][3].  Every stage is one launch over all
+ * instances (uneven instance sizes are balanced by the work-item split), so
+ * a captured run is one graph launch for the whole batch.  All plans must
+ * live on the same device. */
+int scw_batch_exec_create(int n, scw_plan* const* plans, const double* xs, size_t nx, const double* ys, size_t ny,
+                          const double* const* zs, const size_t* nz, double region_tol, int dtype, scw_exec** exec);
+/* Launch shape of the GEMM form: the target number of coefficient work
+ * items (wave-sized) and of main-GEMM workgroups; 0 keeps the default.
+ * Rebuilds the work lists (drops a captured graph).  Used by the autotuner. */
+int scw_exec_set_launch(scw_exec* exec, int64_t coef_items, int64_t main_blocks);
 /* Capture one run into a hipGraph and replay it with scw_exec_graph_launch. */
 int scw_exec_graph_capture(scw_exec* exec, void* out_device, void* stream);
 int scw_exec_graph_launch(scw_exec* exec, void* stream);

@@ -274,3 +274,25 @@ def config5(nx=1024, ny=1024, nz=512, seed=None):
     """16-layer stack, 31x31 pupil, NA 0.7, lambda 0.6; grid through the stack with 2 um margins."""
     cw, z = config5_structure(seed)
     return cw, (axis(-10.0, 10.0, nx), axis(-10.0, 10.0, ny), axis(z[0] - 2.0, z[-1] + 2.0, nz))
+
+
+def config4_instances(n=64, seed=None, orders=(11, 21, 31, 41), size=(64.0, 64.0, 4.0)):
+    """Parameters of the batched sweep: n instances with lambda ~ U[0.4, 1.0] um,
+    incidence ~ U[0, 40] deg and an order count drawn uniformly from `orders`
+    (per axis).  Each instance is one tile-sample slab (one internal region of
+    4 MF^2 eigenmode waves) of size 64 x 64 x 4 um centred at z = 0."""
+    rng = np.random.default_rng(BASE_SEED + 4 if seed is None else seed)
+    return [dict(M=int(rng.choice(orders)), lam=float(rng.uniform(0.4, 1.0)), angle=float(rng.uniform(0.0, 40.0)),
+                 seed=int(rng.integers(1 << 30)), size=size) for _ in range(n)]
+
+
+def config4_structure(p):
+    return tile_sample_structure(M=p["M"], Z=1, size=p["size"], lam=p["lam"], seed=p["seed"],
+                                 incidence_deg=p["angle"])
+
+
+def config4_grid(nx=256, ny=256, nz=64, size=(64.0, 64.0, 4.0)):
+    """256 x 256 x 64 per instance, z inside the slab."""
+    h = size[2] / 2
+    return (axis(-size[0] / 2, size[0] / 2, nx), axis(-size[1] / 2, size[1] / 2, ny),
+            axis(-h + 0.05, h - 0.05, nz))

@@ -96,48 +96,77 @@ static int prep_direct(scw_exec* ex, const double* xs, const double* ys, const d
     return SCW_OK;
 }
 
-static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const double* zs) {
-    const scw_plan* P = ex->plan;
-    const int L = P->layers;
-    const int nx = (int)ex->nx, ny = (int)ex->ny, nz = (int)ex->nz;
+// One grid z of a GEMM-form run: the plan and region it reads and its z.
+struct ZEntry {
+    const scw_plan* plan;
+    int region;
+    double z;
+};
+
+static inline bool region_ok(const cd v) {
+    return std::isfinite(v.real()) && std::isfinite(v.imag()) && std::abs(v) <= 1e30;
+}
+
+// Builds every GEMM-form table of a run over the z entries ze (one per output
+// z-slice, in output order).  Regions are identified by (plan, region) pairs,
+// so a run may mix structures (batches).
+static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std::vector<ZEntry>& ze) {
+    const int nx = (int)ex->nx, ny = (int)ex->ny, nz = (int)ze.size();
     const bool is16 = ex->dtype != SCW_DTYPE_FP32;
     const int kq = is16 ? 32 : 16;  // K granule: one MFMA (16-bit) or one 16-byte Y load of four k-steps (FP32)
-    std::vector<char> used(L + 1, 0);
-    for (int z = 0; z < nz; ++z) used[ex->zregion[z]] = 1;
+    const int ytiles = (ny + 15) / 16;
 
-    std::vector<RegionDev> rd(L + 1);
+    // Virtual regions: distinct (plan, region) pairs in order of first use.
+    std::vector<std::pair<const scw_plan*, int>> vreg;
+    std::vector<int> zv(nz);
+    for (int z = 0; z < nz; ++z) {
+        int v = -1;
+        for (int k = (int)vreg.size() - 1; k >= 0; --k)
+            if (vreg[k].first == ze[z].plan && vreg[k].second == ze[z].region) {
+                v = k;
+                break;
+            }
+        if (v < 0) {
+            v = (int)vreg.size();
+            vreg.push_back({ze[z].plan, ze[z].region});
+        }
+        zv[z] = v;
+    }
+    const int NV = (int)vreg.size();
+    std::vector<RegionDev> rd(NV);
     std::memset(rd.data(), 0, rd.size() * sizeof(RegionDev));
     std::vector<float> X;  // float2 pairs
-    std::vector<int64_t> yfrag_off(L + 1, 0);
+    std::vector<int64_t> yfrag_off(NV, 0);
     std::vector<float> yf32;
     std::vector<uint16_t> yhi, ylo;
-    const int ytiles = (ny + 15) / 16;
     int kmax = 0;
     ex->my_max = 0;
-    for (int r = 0; r <= L; ++r) {
-        if (!used[r]) continue;
+    for (int v = 0; v < NV; ++v) {
+        const scw_plan* P = vreg[v].first;
+        const int r = vreg[v].second;
         const RegionPlan& R = P->regions[r];
-        RegionDev& d = rd[r];
-        d.ell_off = R.ell_off;
-        d.ellc_off = R.ellc_off;
+        if (!R.separable)
+            return fail(SCW_ERR_UNSUPPORTED, "GEMM form: region %d is not separable (%s)", r, R.why_not.c_str());
+        RegionDev& d = rd[v];
+        d.ellA = P->d_ellA.as<float2>() + R.ell_off * 3;
+        d.ellC = R.ellc_off >= 0 ? P->d_ellC.as<int32_t>() + R.ellc_off : nullptr;
         d.cells = R.mx * R.my;
         d.width = R.width;
         d.mc = R.mc;
         d.dense = R.dense ? 1 : 0;
         d.mx = R.mx;
         d.my = R.my;
-        d.nslots = 1;  // set below, once the number of work items is known
+        d.nslots = 1;
         ex->my_max = std::max(ex->my_max, R.my);
         // X_r[a][x] = exp(i kx_a x)
         d.x_off = (int64_t)X.size() / 2;
         X.resize(X.size() + (size_t)2 * R.mx * nx);
         for (int a = 0; a < R.mx; ++a)
             for (int i = 0; i < nx; ++i) {
-                const cd v = std::exp(cd(0, 1) * R.kx[a] * xs[i]);
-                if (!std::isfinite(v.real()) || !std::isfinite(v.imag()) || std::abs(v) > 1e30)
-                    return fail(SCW_ERR_UNSUPPORTED, "region %d: exp(i kx x) outside FP32 range", r);
-                X[2 * (d.x_off + (int64_t)a * nx + i)] = (float)v.real();
-                X[2 * (d.x_off + (int64_t)a * nx + i) + 1] = (float)v.imag();
+                const cd val = std::exp(cd(0, 1) * R.kx[a] * xs[i]);
+                if (!region_ok(val)) return fail(SCW_ERR_UNSUPPORTED, "region %d: exp(i kx x) outside FP32 range", r);
+                X[2 * (d.x_off + (int64_t)a * nx + i)] = (float)val.real();
+                X[2 * (d.x_off + (int64_t)a * nx + i) + 1] = (float)val.imag();
             }
         // Y fragments: Yp[y][2b] = Re exp(i ky_b y), Yp[y][2b+1] = Im.
         const int kp = R.my > 0 ? round_up(2 * R.my, kq) : 0;
@@ -145,16 +174,15 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const dou
         std::vector<double> Yp((size_t)ny * std::max(kp, 1), 0.0);
         for (int j = 0; j < ny; ++j)
             for (int b = 0; b < R.my; ++b) {
-                const cd v = std::exp(cd(0, 1) * R.ky[b] * ys[j]);
-                if (!std::isfinite(v.real()) || !std::isfinite(v.imag()) || std::abs(v) > 1e30)
-                    return fail(SCW_ERR_UNSUPPORTED, "region %d: exp(i ky y) outside FP32 range", r);
-                Yp[(size_t)j * kp + 2 * b] = v.real();
-                Yp[(size_t)j * kp + 2 * b + 1] = v.imag();
+                const cd val = std::exp(cd(0, 1) * R.ky[b] * ys[j]);
+                if (!region_ok(val)) return fail(SCW_ERR_UNSUPPORTED, "region %d: exp(i ky y) outside FP32 range", r);
+                Yp[(size_t)j * kp + 2 * b] = val.real();
+                Yp[(size_t)j * kp + 2 * b + 1] = val.imag();
             }
         auto yval = [&](int y, int k) { return (y < ny && k < kp) ? Yp[(size_t)y * kp + k] : 0.0; };
         if (!is16) {
-            yfrag_off[r] = (int64_t)yf32.size();
             // [yt][q][lane][e]: lane l holds k = 4 (4q + e) + (l >> 4) of row 16 yt + (l & 15)
+            yfrag_off[v] = (int64_t)yf32.size();
             const int K16 = kp / 16;
             for (int yt = 0; yt < ytiles; ++yt)
                 for (int q = 0; q < K16; ++q)
@@ -162,49 +190,65 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const dou
                         for (int e = 0; e < 4; ++e)
                             yf32.push_back((float)yval(16 * yt + (l & 15), 4 * (4 * q + e) + (l >> 4)));
         } else {
-            yfrag_off[r] = (int64_t)yhi.size();
+            // [yt][s][lane][e]: lane l holds k = 32 s + 8 (l >> 4) + e of row 16 yt + (l & 15)
+            yfrag_off[v] = (int64_t)yhi.size();
             const int K32 = kp / 32;
             for (int yt = 0; yt < ytiles; ++yt)
-                for (int s = 0; s < K32; ++s)
+                for (int s2 = 0; s2 < K32; ++s2)
                     for (int l = 0; l < 64; ++l)
                         for (int e = 0; e < 8; ++e) {
-                            const double v = yval(16 * yt + (l & 15), 32 * s + 8 * (l >> 4) + e);
+                            const double val = yval(16 * yt + (l & 15), 32 * s2 + 8 * (l >> 4) + e);
                             if (ex->dtype == SCW_DTYPE_FP16) {
-                                yhi.push_back(f16_rne((float)v));
+                                yhi.push_back(f16_rne((float)val));
                                 ylo.push_back(0);
                             } else {
-                                const uint16_t h = bf16_rne((float)v);
+                                const uint16_t h = bf16_rne((float)val);
                                 yhi.push_back(h);
-                                ylo.push_back(bf16_rne((float)(v - (double)bf16_f32(h))));
+                                ylo.push_back(bf16_rne((float)(val - (double)bf16_f32(h))));
                             }
                         }
         }
     }
-    if (!is16) {
-        ex->kmax = kmax <= 16 ? 16 : kmax <= 32 ? 32 : kmax <= 64 ? 64 : 128;
-    } else {
-        ex->kmax = kmax <= 32 ? 32 : kmax <= 64 ? 64 : kmax <= 96 ? 96 : 128;
-    }
     if (kmax > 128) return fail(SCW_ERR_UNSUPPORTED, "GEMM form supports at most 64 distinct ky per region");
+    if (!is16)
+        ex->kmax = kmax <= 16 ? 16 : kmax <= 32 ? 32 : kmax <= 64 ? 64 : 128;
+    else
+        ex->kmax = kmax <= 32 ? 32 : kmax <= 64 ? 64 : kmax <= 96 ? 96 : 128;
 
-    // Term-slice split: enough wave-sized work items to cover every SIMD a few
-    // times over, with at least 16 terms per slice.
+    // Runs: consecutive z of one virtual region.  Dense eigenmode regions
+    // contract on MFMA when the dtype is 16-bit (runs of up to 64 z, every
+    // term read once per run); other regions use the VALU ELL contraction
+    // (runs of up to 8 z).
+    struct Run {
+        int z0, z1, v;
+        bool modal;
+    };
+    std::vector<Run> runs;
+    for (int z = 0; z < nz;) {
+        const int v = zv[z];
+        int zend = z + 1;  // maximal stretch of consecutive z in region v
+        while (zend < nz && zv[zend] == v) ++zend;
+        // The MFMA contraction pays off once the VALU path (8 z per pass)
+        // would read the terms more than once.
+        const bool modal = is16 && rd[v].dense && rd[v].width > 0 && zend - z > 8;
+        const int cap = modal ? 64 : 8;
+        for (int z0 = z; z0 < zend; z0 += cap) runs.push_back({z0, std::min(zend, z0 + cap), v, modal});
+        z = zend;
+    }
+    // Slices of the term (or mode) range: enough wave items to cover every
+    // SIMD several times, with at least 16 terms per slice.
     {
         int64_t base = 0;
-        for (int z = 0; z < nz;) {
-            const int r = ex->zregion[z];
-            int e = z + 1;
-            while (e < nz && e - z < 8 && ex->zregion[e] == r) ++e;
-            base += (rd[r].cells + 63) / 64;
-            z = e;
+        for (const Run& rn : runs) {
+            const RegionDev& d = rd[rn.v];
+            base += rn.modal ? (6LL * d.cells + 63) / 64 : (d.cells + 63) / 64;
         }
-        const int64_t target = tunable("SCW_COEF_ITEMS", 4096);
+        const int64_t target = ex->tune_coef_items > 0 ? ex->tune_coef_items : tunable("SCW_COEF_ITEMS", 4096);
         const int want = base > 0 ? (int)std::min<int64_t>(64, (target + base - 1) / base) : 1;
-        for (int r = 0; r <= L; ++r)
-            if (used[r]) {
-                rd[r].nslots = std::max(1, std::min(want, rd[r].width / 16));
-                ex->nslots_max = std::max(ex->nslots_max, rd[r].nslots);
-            }
+        for (int v = 0; v < NV; ++v) {
+            rd[v].nslots = std::max(1, std::min(want, rd[v].width / 16));
+            ex->nslots_max = std::max(ex->nslots_max, rd[v].nslots);
+        }
     }
 
     // Per-z metadata, Z rows, C and U offsets.
@@ -214,103 +258,75 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const dou
     double f_coef = 0, f_u = 0, f_main = 0;
     double b_coef = 0, b_u = 0, b_main = 24.0 * (double)nx * ny * nz;
     for (int z = 0; z < nz; ++z) {
-        const int r = ex->zregion[z];
-        const RegionPlan& R = P->regions[r];
+        const int v = zv[z];
+        const RegionPlan& R = vreg[v].first->regions[vreg[v].second];
         ZMeta& m = zm[z];
-        m.region = r;
+        m.region = v;
         m.my = R.my;
         m.mx = R.mx;
         m.kp = R.my > 0 ? round_up(2 * R.my, kq) : 0;
         m.u_off = u_off;
-        m.yfrag_off = yfrag_off[r];
+        m.yfrag_off = yfrag_off[v];
         m.zrow_off = (int64_t)Z.size() / 2;
         m.c_off = c_off;
         u_off += (int64_t)R.my * nx * 3;
         c_off += (int64_t)R.mx * R.my * 3;
         for (int c = 0; c < R.mc; ++c) {
-            const cd v = std::exp(cd(0, 1) * R.kz[c] * (zs[z] - R.zref));
-            if (!std::isfinite(v.real()) || !std::isfinite(v.imag()) || std::abs(v) > 1e30)
-                return fail(SCW_ERR_UNSUPPORTED, "region %d: exp(i kz (z - zref)) outside FP32 range at z=%g", r,
-                            zs[z]);
-            Z.push_back((float)v.real());
-            Z.push_back((float)v.imag());
+            const cd val = std::exp(cd(0, 1) * R.kz[c] * (ze[z].z - R.zref));
+            if (!region_ok(val))
+                return fail(SCW_ERR_UNSUPPORTED, "region %d: exp(i kz (z - zref)) outside FP32 range at z=%g",
+                            vreg[v].second, ze[z].z);
+            Z.push_back((float)val.real());
+            Z.push_back((float)val.imag());
         }
         f_coef += 24.0 * (double)R.n_terms;
         f_u += 24.0 * (double)R.mx * R.my * nx;
         f_main += 24.0 * (double)R.my * nx * ny;
-        const double cbytes = 24.0 * (double)R.mx * R.my;   // C_z, one slot
-        const double ubytes = 24.0 * (double)R.my * nx;      // U_z
-        b_coef += cbytes * rd[r].nslots;
-        b_u += cbytes * rd[r].nslots + ubytes + 8.0 * R.mx * nx;
+        const double cbytes = 24.0 * (double)R.mx * R.my;  // C_z, one slot
+        const double ubytes = 24.0 * (double)R.my * nx;     // U_z
+        b_coef += cbytes * rd[v].nslots;
+        b_u += cbytes * rd[v].nslots + ubytes + 8.0 * R.mx * nx;
         b_main += ubytes;
     }
     ex->c_elems = c_off;
 
-    // Coefficient work items: runs of <= 8 consecutive z in one region.
+    // Work items.
     ex->items.clear();
-    std::vector<std::pair<int, int>> runs;  // (first z, first item) of each run
-    for (int z = 0; z < nz;) {
-        const int r = ex->zregion[z];
-        int e = z + 1;
-        while (e < nz && e - z < 8 && ex->zregion[e] == r) ++e;
-        runs.push_back({z, (int)ex->items.size()});
-        const RegionDev& d = rd[r];
-        if (d.cells > 0 && d.width > 0) {
+    ex->mitems.clear();
+    for (const Run& rn : runs) {
+        const RegionDev& d = rd[rn.v];
+        if (d.cells == 0 || d.width == 0) continue;
+        b_coef += 24.0 * (double)d.cells * d.width + 8.0 * d.mc * (rn.z1 - rn.z0);  // terms once per run, Z rows
+        if (rn.modal) {
+            const int per = round_up((d.width + d.nslots - 1) / d.nslots, 16);
+            for (int n0 = 0; n0 < 6 * d.cells; n0 += 64)
+                for (int s2 = 0; s2 < d.nslots; ++s2) {
+                    ModalItem it;
+                    it.region = rn.v;
+                    it.z0 = rn.z0;
+                    it.nz = rn.z1 - rn.z0;
+                    it.n0 = n0;
+                    it.c0 = std::min(d.width, s2 * per);
+                    it.c1 = std::min(d.width, (s2 + 1) * per);
+                    it.slot = s2;
+                    it.pad = 0;
+                    ex->mitems.push_back(it);
+                }
+        } else {
             const int per = (d.width + d.nslots - 1) / d.nslots;
             for (int c0 = 0; c0 < d.cells; c0 += 64)
-                for (int s = 0; s < d.nslots; ++s) {
+                for (int s2 = 0; s2 < d.nslots; ++s2) {
                     CoefItem it;
-                    it.region = r;
+                    it.region = rn.v;
                     it.cell0 = c0;
-                    it.j0 = std::min(d.width, s * per);
-                    it.j1 = std::min(d.width, (s + 1) * per);
-                    it.z0 = z;
-                    it.nz = e - z;
-                    it.slot = s;
+                    it.j0 = std::min(d.width, s2 * per);
+                    it.j1 = std::min(d.width, (s2 + 1) * per);
+                    it.z0 = rn.z0;
+                    it.nz = rn.z1 - rn.z0;
+                    it.slot = s2;
                     it.pad = 0;
                     ex->items.push_back(it);
                 }
-            b_coef += 24.0 * (double)d.cells * d.width + 8.0 * d.mc * (e - z);  // ELL amplitudes once per run, Z rows
-        }
-        z = e;
-    }
-    // Z groups: contiguous runs of about nz / ng slices each.
-    {
-        const int nruns = (int)runs.size();
-        // Forking several groups shrinks every launch; measured on the tile
-        // sample (21 runs, 128 z) four groups were slower than one
-        // (0.214 vs 0.197 ms), so groups are only used for grids whose
-        // single-group launches already fill the GPU several times over.
-        const int64_t waves_main = (int64_t)((6LL * nx + 255) / 256) * nz * 4;
-        int ng = (nruns >= 2 && waves_main >= 64 * 1024 && (int64_t)ex->items.size() >= 16384)
-                     ? std::min(2, nruns) : 1;
-        if (const char* env = std::getenv("SCW_ZGROUPS")) {  // override, used by the tests
-            const int v = std::atoi(env);
-            if (v >= 1) ng = std::min(v, std::max(1, nruns));
-        }
-        const int target = (nz + ng - 1) / ng;
-        ex->groups.clear();
-        ZGroup g{0, 0, 0, 0};
-        for (int k = 0; k < nruns; ++k) {
-            const int zend = k + 1 < nruns ? runs[k + 1].first : nz;
-            const int iend = k + 1 < nruns ? runs[k + 1].second : (int)ex->items.size();
-            g.z1 = zend;
-            g.i1 = iend;
-            if (g.z1 - g.z0 >= target || k + 1 == nruns) {
-                ex->groups.push_back(g);
-                g = ZGroup{zend, zend, iend, iend};
-            }
-        }
-        if (ex->groups.size() > 1) {
-            SCW_HIP(hipEventCreateWithFlags(&ex->ev_fork, hipEventDisableTiming));
-            for (size_t i = 0; i < ex->groups.size(); ++i) {
-                hipStream_t bs = nullptr;
-                SCW_HIP(hipStreamCreateWithFlags(&bs, hipStreamNonBlocking));
-                ex->bstreams.push_back(bs);
-                hipEvent_t ev = nullptr;
-                SCW_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-                ex->ev_join.push_back(ev);
-            }
         }
     }
     ex->stage_flops[1] = f_coef;
@@ -321,14 +337,15 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const dou
     ex->stage_bytes[3] = b_main;
     ex->stage_flops[0] = ex->flops = f_coef + f_u + f_main;
     ex->stage_bytes[0] = ex->bytes = b_coef + b_u + b_main;
-    ex->launches = (ex->items.empty() ? 0 : 1) + (ex->my_max > 0 ? 1 : 0) + 1;
+    ex->launches = (ex->items.empty() ? 0 : 1) + (ex->mitems.empty() ? 0 : 1) + (ex->my_max > 0 ? 1 : 0) + 1;
 
     const int gx = (int)((6LL * nx + 255) / 256);
-    // Each wave keeps its B-operand fragment (from U_z) in registers for all of
-    // its y tiles, so split y across workgroups only as far as needed to give
-    // every CU a couple of workgroups.
-    const int64_t mtarget = tunable("SCW_MAIN_BLOCKS", 512);
-    int ysplit = (int)std::min<int64_t>(ytiles, std::max<int64_t>(1, (mtarget + (int64_t)gx * nz - 1) / ((int64_t)gx * nz)));
+    // Each wave keeps its A-operand fragment (from U_z) in registers for all
+    // of its y tiles, so split y across workgroups only as far as needed to
+    // give every CU a couple of workgroups.
+    const int64_t mtarget = ex->tune_main_blocks > 0 ? ex->tune_main_blocks : tunable("SCW_MAIN_BLOCKS", 512);
+    const int ysplit =
+        (int)std::min<int64_t>(ytiles, std::max<int64_t>(1, (mtarget + (int64_t)gx * nz - 1) / ((int64_t)gx * nz)));
     ex->ytiles_per_split = (ytiles + ysplit - 1) / ysplit;
     ex->ysplit = (ytiles + ex->ytiles_per_split - 1) / ex->ytiles_per_split;
 
@@ -336,6 +353,7 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const dou
     if ((st = ex->d_regions.upload(rd.data(), rd.size() * sizeof(RegionDev))) != SCW_OK) return st;
     if ((st = ex->d_zmeta.upload(zm.data(), zm.size() * sizeof(ZMeta))) != SCW_OK) return st;
     if ((st = ex->d_items.upload(ex->items.data(), ex->items.size() * sizeof(CoefItem))) != SCW_OK) return st;
+    if ((st = ex->d_mitems.upload(ex->mitems.data(), ex->mitems.size() * sizeof(ModalItem))) != SCW_OK) return st;
     if ((st = ex->d_Z.upload(Z.data(), Z.size() * sizeof(float))) != SCW_OK) return st;
     if ((st = ex->d_X.upload(X.data(), X.size() * sizeof(float))) != SCW_OK) return st;
     if ((st = ex->d_C.alloc((size_t)ex->nslots_max * std::max<int64_t>(c_off, 1) * 8)) != SCW_OK) return st;
@@ -354,14 +372,21 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const dou
     return SCW_OK;
 }
 
+// The z entries of an exec: plan, region and z of every output slice.
+static std::vector<ZEntry> entries(const scw_exec* ex) {
+    std::vector<ZEntry> ze(ex->hz.size());
+    for (size_t z = 0; z < ze.size(); ++z) {
+        const scw_plan* P = ex->batch.empty() ? ex->plan : ex->batch[ex->hzp[z]];
+        ze[z] = {P, ex->zregion[z], ex->hz[z]};
+    }
+    return ze;
+}
+
 static void exec_free(scw_exec* ex) {
     if (ex->graph_exec) (void)hipGraphExecDestroy(ex->graph_exec);
     if (ex->graph) (void)hipGraphDestroy(ex->graph);
     if (ex->graph_stream) (void)hipStreamDestroy(ex->graph_stream);
-    for (hipStream_t bs : ex->bstreams) (void)hipStreamDestroy(bs);
-    for (hipEvent_t ev : ex->ev_join) (void)hipEventDestroy(ev);
-    if (ex->ev_fork) (void)hipEventDestroy(ex->ev_fork);
-    for (DevBuf* b : {&ex->d_dx, &ex->d_dy, &ex->d_z, &ex->d_zslab, &ex->d_items, &ex->d_regions, &ex->d_zmeta,
+    for (DevBuf* b : {&ex->d_dx, &ex->d_dy, &ex->d_z, &ex->d_zslab, &ex->d_items, &ex->d_mitems, &ex->d_regions, &ex->d_zmeta,
                       &ex->d_Z, &ex->d_X, &ex->d_C, &ex->d_U, &ex->d_Yfrag, &ex->d_out})
         b->release();
     delete ex;
@@ -432,7 +457,11 @@ int scw_exec_create(scw_plan* plan, const double* xs, size_t nx, const double* y
     }
     int st;
     try {
-        st = form == SCW_FORM_DIRECT ? prep_direct(ex, xs, ys, zs) : prep_gemm(ex, xs, ys, zs);
+        ex->hx.assign(xs, xs + nx);
+        ex->hy.assign(ys, ys + ny);
+        ex->hz.assign(zs, zs + nz);
+        ex->hzp.assign(nz, 0);
+        st = form == SCW_FORM_DIRECT ? prep_direct(ex, xs, ys, zs) : prep_gemm(ex, xs, ys, entries(ex));
     } catch (const std::bad_alloc&) {
         st = fail(SCW_ERR_NOMEM, "host allocation failed while building grid tables");
     }
@@ -442,6 +471,100 @@ int scw_exec_create(scw_plan* plan, const double* xs, size_t nx, const double* y
     }
     *out = ex;
     return SCW_OK;
+}
+
+int scw_batch_exec_create(int n, scw_plan* const* plans, const double* xs, size_t nx, const double* ys, size_t ny,
+                          const double* const* zs, const size_t* nz, double region_tol, int dtype, scw_exec** out) {
+    if (!out) return fail(SCW_ERR_ARG, "exec output is NULL");
+    *out = nullptr;
+    if (n <= 0 || !plans || !zs || !nz) return fail(SCW_ERR_ARG, "empty or NULL batch");
+    if (!xs || !ys || nx == 0 || ny == 0) return fail(SCW_ERR_ARG, "empty x / y axes");
+    if (nx > (1u << 24) || ny > (1u << 24)) return fail(SCW_ERR_ARG, "grid too large (nx, ny <= 2^24)");
+    if (dtype < SCW_DTYPE_FP32 || dtype > SCW_DTYPE_FP16) return fail(SCW_ERR_ARG, "unknown dtype %d", dtype);
+    if (!std::isfinite(region_tol)) return fail(SCW_ERR_ARG, "region_tol is not finite");
+    size_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        if (!plans[i] || !zs[i] || nz[i] == 0) return fail(SCW_ERR_ARG, "instance %d: NULL plan or empty z axis", i);
+        if (plans[i]->device != plans[0]->device) return fail(SCW_ERR_ARG, "instance %d: plans on different devices", i);
+        for (size_t k = 0; k < nz[i]; ++k)
+            if (!std::isfinite(zs[i][k])) return fail(SCW_ERR_ARG, "instance %d: z[%zu] is not finite", i, k);
+        total += nz[i];
+    }
+    if (total > 65535) return fail(SCW_ERR_ARG, "batch has more than 65535 z-slices");
+    if ((double)nx * ny * total * 24.0 > 1.0e12) return fail(SCW_ERR_ARG, "batch output exceeds 1 TB");
+    for (size_t i = 0; i < nx; ++i)
+        if (!std::isfinite(xs[i])) return fail(SCW_ERR_ARG, "xs[%zu] is not finite", i);
+    for (size_t i = 0; i < ny; ++i)
+        if (!std::isfinite(ys[i])) return fail(SCW_ERR_ARG, "ys[%zu] is not finite", i);
+
+    scw_exec* ex = new (std::nothrow) scw_exec();
+    if (!ex) return fail(SCW_ERR_NOMEM, "host allocation failed");
+    int st = SCW_OK;
+    try {
+        ex->plan = plans[0];
+        ex->batch.assign(plans, plans + n);
+        ex->nx = (int64_t)nx;
+        ex->ny = (int64_t)ny;
+        ex->nz = (int64_t)total;
+        ex->dtype = dtype;
+        ex->form = SCW_FORM_GEMM;
+        ex->hx.assign(xs, xs + nx);
+        ex->hy.assign(ys, ys + ny);
+        double units = 0.0;
+        for (int i = 0; i < n; ++i)
+            for (size_t k = 0; k < nz[i]; ++k) {
+                const int r = region_of(zs[i][k], plans[i]->zl, region_tol);
+                ex->hz.push_back(zs[i][k]);
+                ex->hzp.push_back(i);
+                ex->zregion.push_back(r);
+                const RegionPlan& R = plans[i]->regions[r];
+                units += (double)nx * ny * (double)(R.wend - R.wbegin);
+            }
+        ex->units = units;
+    } catch (const std::bad_alloc&) {
+        delete ex;
+        return fail(SCW_ERR_NOMEM, "host allocation failed");
+    }
+    DeviceGuard g(plans[0]->device);
+    if (g.st != SCW_OK) {
+        delete ex;
+        return g.st;
+    }
+    try {
+        st = prep_gemm(ex, xs, ys, entries(ex));
+    } catch (const std::bad_alloc&) {
+        st = fail(SCW_ERR_NOMEM, "host allocation failed while building grid tables");
+    }
+    if (st != SCW_OK) {
+        exec_free(ex);
+        return st;
+    }
+    *out = ex;
+    return SCW_OK;
+}
+
+int scw_exec_set_launch(scw_exec* ex, int64_t coef_items, int64_t main_blocks) {
+    if (!ex) return fail(SCW_ERR_ARG, "exec is NULL");
+    if (coef_items < 0 || main_blocks < 0) return fail(SCW_ERR_ARG, "negative launch parameter");
+    if (ex->form != SCW_FORM_GEMM) return SCW_OK;  // the direct form has no tunable launch shape
+    DeviceGuard g(ex->plan->device);
+    if (g.st != SCW_OK) return g.st;
+    if (ex->graph_exec) {
+        (void)hipGraphExecDestroy(ex->graph_exec);
+        ex->graph_exec = nullptr;
+    }
+    if (ex->graph) {
+        (void)hipGraphDestroy(ex->graph);
+        ex->graph = nullptr;
+    }
+    ex->tune_coef_items = coef_items;
+    ex->tune_main_blocks = main_blocks;
+    ex->nslots_max = 1;
+    try {
+        return prep_gemm(ex, ex->hx.data(), ex->hy.data(), entries(ex));
+    } catch (const std::bad_alloc&) {
+        return fail(SCW_ERR_NOMEM, "host allocation failed while building grid tables");
+    }
 }
 
 int scw_exec_run(scw_exec* ex, void* out_device, void* stream) {

@@ -87,8 +87,7 @@ __device__ inline void coef_loop(float (&acc)[8][6], const float2* __restrict__ 
 
 __global__ __launch_bounds__(256) void coef_kernel(const CoefItem* __restrict__ items, int nitems,
                                                    const RegionDev* __restrict__ regions,
-                                                   const ZMeta* __restrict__ zmeta, const float2* __restrict__ ellA,
-                                                   const int32_t* __restrict__ ellC, const float2* __restrict__ Z,
+                                                   const ZMeta* __restrict__ zmeta, const float2* __restrict__ Z,
                                                    int64_t c_elems, float2* __restrict__ C) {
     const int lane = threadIdx.x & 63;
     const int it = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
@@ -108,8 +107,8 @@ __global__ __launch_bounds__(256) void coef_kernel(const CoefItem* __restrict__ 
 #pragma unroll
         for (int c = 0; c < 6; ++c) acc[zi][c] = 0.0f;
 
-    const float2* Ab = ellA + (R.ell_off + cl) * 3;
-    const int32_t* Cb = ellC + R.ellc_off + cl;
+    const float2* Ab = R.ellA + (int64_t)cl * 3;
+    const int32_t* Cb = R.dense ? nullptr : R.ellC + cl;
     const int64_t stride = (int64_t)R.cells;
 #define SCW_COEF_CASE(N)                                                                   \
     case N:                                                                                \
@@ -445,33 +444,178 @@ static void launch_main(const scw_exec* ex, dim3 grid, hipStream_t st, int ytps,
     }
 }
 
-// The three stages of one z group (z in [z0, z1), coefficient items [i0, i1)).
-static int launch_group(const scw_exec* ex, const ZGroup& g, float* out, hipStream_t st, int stage) {
-    const int nitems = g.i1 - g.i0;
-    const int nzg = g.z1 - g.z0;
-    if (nzg <= 0) return SCW_OK;
-    if (nitems > 0 && (stage == 0 || stage == 1)) {
-        hipLaunchKernelGGL(coef_kernel, dim3((unsigned)((nitems + 3) / 4)), dim3(256), 0, st,
-                           ex->d_items.as<CoefItem>() + g.i0, nitems, ex->d_regions.as<RegionDev>(),
-                           ex->d_zmeta.as<ZMeta>(), ex->plan->d_ellA.as<float2>(), ex->plan->d_ellC.as<int32_t>(),
-                           ex->d_Z.as<float2>(), ex->c_elems, ex->d_C.as<float2>());
-        SCW_HIP(hipGetLastError());
+// ---------------------------------------------------------------------------
+// MFMA mode contraction for dense eigenmode regions (16-bit operand modes):
+//   C_z[n] = sum_c Z[z][c] A[c][n],   n = real column (cell, comp, re|im)
+// as a real GEMM with the complex product folded into K (k = 2c + s):
+//   A'[2c][n] = A re|im as stored, A'[2c+1][n] = (i A) re|im,
+//   Z'[z][2c] = Re Z, Z'[z][2c+1] = Im Z.
+// MFMA rows are n (4 consecutive per lane), columns z (16 per tile, up to 4
+// tiles), K 32 per step = 16 modes.  Every term is read once per run.
+// ---------------------------------------------------------------------------
+template <int MODE, int NZT>
+__device__ inline void modal_body(const ModalItem& I, const RegionDev& R, const ZMeta* __restrict__ zmeta,
+                                  const float2* __restrict__ Z, int64_t c_elems, float2* __restrict__ C) {
+    typedef typename Op16<MODE>::vec vec;
+    typedef typename Op16<MODE>::elem elem;
+    const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    const int64_t ncols = 6LL * R.cells;
+    const float* Ab = reinterpret_cast<const float*>(R.ellA);
+    int64_t zr[NZT];
+    bool zok[NZT];
+#pragma unroll
+    for (int zt = 0; zt < NZT; ++zt) {
+        const int zz = 16 * zt + li;
+        zok[zt] = zz < I.nz;
+        zr[zt] = zmeta[I.z0 + min(zz, I.nz - 1)].zrow_off;
+    }
+    floatx4 acc[4][NZT];
+#pragma unroll
+    for (int it = 0; it < 4; ++it)
+#pragma unroll
+        for (int zt = 0; zt < NZT; ++zt) acc[it][zt] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    // Raw operands of one 16-mode step; the next step's loads are issued
+    // before the current step's conversions and MFMAs.
+    float2 ra[4][4], rz[NZT][4];
+    auto load_step = [&](int cb) {
+#pragma unroll
+        for (int it = 0; it < 4; ++it) {
+            const int64_t n = I.n0 + 16 * it + li;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int c = cb + 4 * lk + q;
+                ra[it][q] = (n < ncols && c < I.c1)
+                                ? *reinterpret_cast<const float2*>(Ab + (int64_t)c * ncols + (n & ~(int64_t)1))
+                                : make_float2(0.0f, 0.0f);
+            }
+        }
+#pragma unroll
+        for (int zt = 0; zt < NZT; ++zt)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int c = cb + 4 * lk + q;
+                rz[zt][q] = (zok[zt] && c < I.c1) ? Z[zr[zt] + c] : make_float2(0.0f, 0.0f);
+            }
+    };
+    if (I.c0 < I.c1) load_step(I.c0);
+    for (int cb = I.c0; cb < I.c1; cb += 16) {
+        vec ah[4], al[4], zh[NZT], zl[NZT];
+#pragma unroll
+        for (int it = 0; it < 4; ++it) {
+            const bool odd = ((I.n0 + 16 * it + li) & 1) != 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float2 u = ra[it][q];
+                const float v0 = odd ? u.y : u.x;   // k = 2c:   A re (n even) / A im (n odd)
+                const float v1 = odd ? u.x : -u.y;  // k = 2c+1: -A im (n even) / A re (n odd)
+                const elem h0 = (elem)v0, h1 = (elem)v1;
+                ah[it][2 * q] = h0;
+                ah[it][2 * q + 1] = h1;
+                if constexpr (MODE == 1) {
+                    al[it][2 * q] = (elem)(v0 - (float)h0);
+                    al[it][2 * q + 1] = (elem)(v1 - (float)h1);
+                }
+            }
+        }
+#pragma unroll
+        for (int zt = 0; zt < NZT; ++zt) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float2 f = rz[zt][q];
+                const elem h0 = (elem)f.x, h1 = (elem)f.y;
+                zh[zt][2 * q] = h0;
+                zh[zt][2 * q + 1] = h1;
+                if constexpr (MODE == 1) {
+                    zl[zt][2 * q] = (elem)(f.x - (float)h0);
+                    zl[zt][2 * q + 1] = (elem)(f.y - (float)h1);
+                }
+            }
+        }
+        if (cb + 16 < I.c1) load_step(cb + 16);
+#pragma unroll
+        for (int it = 0; it < 4; ++it)
+#pragma unroll
+            for (int zt = 0; zt < NZT; ++zt) {
+                if constexpr (MODE == 1) {
+                    acc[it][zt] = mfma16<MODE>(al[it], zh[zt], acc[it][zt]);
+                    acc[it][zt] = mfma16<MODE>(ah[it], zl[zt], acc[it][zt]);
+                }
+                acc[it][zt] = mfma16<MODE>(ah[it], zh[zt], acc[it][zt]);
+            }
+    }
+    // lane holds D[n = n0 + 16 it + 4 lk + r][z = 16 zt + li]
+#pragma unroll
+    for (int zt = 0; zt < NZT; ++zt) {
+        if (!zok[zt]) continue;
+        float* o = reinterpret_cast<float*>(C + (int64_t)I.slot * c_elems + zmeta[I.z0 + 16 * zt + li].c_off);
+#pragma unroll
+        for (int it = 0; it < 4; ++it) {
+            const int64_t n = I.n0 + 16 * it + 4 * lk;
+            if (n + 1 < ncols) *reinterpret_cast<float2*>(o + n) = make_float2(acc[it][zt][0], acc[it][zt][1]);
+            if (n + 3 < ncols) *reinterpret_cast<float2*>(o + n + 2) = make_float2(acc[it][zt][2], acc[it][zt][3]);
+        }
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void modal_kernel(const ModalItem* __restrict__ items, int nitems,
+                                                    const RegionDev* __restrict__ regions,
+                                                    const ZMeta* __restrict__ zmeta, const float2* __restrict__ Z,
+                                                    int64_t c_elems, float2* __restrict__ C) {
+    const int it = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+    if (it >= nitems) return;
+    const ModalItem I = items[it];
+    const RegionDev R = regions[I.region];
+    if (I.nz <= 16) modal_body<MODE, 1>(I, R, zmeta, Z, c_elems, C);
+    else if (I.nz <= 32) modal_body<MODE, 2>(I, R, zmeta, Z, c_elems, C);
+    else modal_body<MODE, 4>(I, R, zmeta, Z, c_elems, C);
+}
+
+// Stages: 1 = coefficient contraction (VALU ELL items and MFMA mode items),
+// 2 = X contraction, 3 = main MFMA GEMM; 0 = all three.
+int launch_gemm(const scw_exec* ex, float* out, hipStream_t st, int stage) {
+    const int nitems = (int)ex->items.size();
+    const int nmitems = (int)ex->mitems.size();
+    if (stage == 0 || stage == 1) {
+        if (nitems > 0) {
+            hipLaunchKernelGGL(coef_kernel, dim3((unsigned)((nitems + 3) / 4)), dim3(256), 0, st,
+                               ex->d_items.as<CoefItem>(), nitems, ex->d_regions.as<RegionDev>(),
+                               ex->d_zmeta.as<ZMeta>(), ex->d_Z.as<float2>(), ex->c_elems, ex->d_C.as<float2>());
+            SCW_HIP(hipGetLastError());
+        }
+        if (nmitems > 0) {
+            const dim3 g((unsigned)((nmitems + 3) / 4));
+            const ModalItem* mi = ex->d_mitems.as<ModalItem>();
+            const RegionDev* rg = ex->d_regions.as<RegionDev>();
+            const ZMeta* zm = ex->d_zmeta.as<ZMeta>();
+            if (ex->dtype == SCW_DTYPE_BF16)
+                hipLaunchKernelGGL(modal_kernel<0>, g, dim3(256), 0, st, mi, nmitems, rg, zm, ex->d_Z.as<float2>(),
+                                   ex->c_elems, ex->d_C.as<float2>());
+            else if (ex->dtype == SCW_DTYPE_BF16X3)
+                hipLaunchKernelGGL(modal_kernel<1>, g, dim3(256), 0, st, mi, nmitems, rg, zm, ex->d_Z.as<float2>(),
+                                   ex->c_elems, ex->d_C.as<float2>());
+            else
+                hipLaunchKernelGGL(modal_kernel<2>, g, dim3(256), 0, st, mi, nmitems, rg, zm, ex->d_Z.as<float2>(),
+                                   ex->c_elems, ex->d_C.as<float2>());
+            SCW_HIP(hipGetLastError());
+        }
     }
     if (ex->my_max > 0 && (stage == 0 || stage == 2)) {
-        hipLaunchKernelGGL(u_kernel, dim3((unsigned)((ex->nx + 255) / 256), (unsigned)ex->my_max, (unsigned)nzg),
+        hipLaunchKernelGGL(u_kernel, dim3((unsigned)((ex->nx + 255) / 256), (unsigned)ex->my_max, (unsigned)ex->nz),
                            dim3(256), 0, st, ex->d_zmeta.as<ZMeta>(), ex->d_regions.as<RegionDev>(),
                            ex->d_C.as<float2>(), ex->c_elems, ex->d_X.as<float2>(), (int)ex->nx, ex->d_U.as<float2>(),
-                           g.z0);
+                           0);
         SCW_HIP(hipGetLastError());
     }
     if (stage != 0 && stage != 3) return SCW_OK;
     const int64_t ncols = 6 * ex->nx;
-    dim3 grid((unsigned)((ncols + 255) / 256), (unsigned)nzg, (unsigned)ex->ysplit);
+    dim3 grid((unsigned)((ncols + 255) / 256), (unsigned)ex->nz, (unsigned)ex->ysplit);
     const int ytps = ex->ytiles_per_split;
     const bool vec = (ncols % 4) == 0;
 #define SCW_MAIN(K)                                                        \
-    if (vec) launch_main<K, true>(ex, grid, st, ytps, out, g.z0);          \
-    else launch_main<K, false>(ex, grid, st, ytps, out, g.z0);
+    if (vec) launch_main<K, true>(ex, grid, st, ytps, out, 0);             \
+    else launch_main<K, false>(ex, grid, st, ytps, out, 0);
     if (ex->dtype == SCW_DTYPE_FP32) {
         switch (ex->kmax) {
             case 16: SCW_MAIN(16) break;
@@ -489,32 +633,6 @@ static int launch_group(const scw_exec* ex, const ZGroup& g, float* out, hipStre
     }
 #undef SCW_MAIN
     SCW_HIP(hipGetLastError());
-    return SCW_OK;
-}
-
-// A full run forks the z groups onto the exec's branch streams so that one
-// group's read-bound coefficient stage overlaps another group's write-bound
-// MFMA stage, and joins them back onto the caller's stream.  Captured into a
-// hipGraph this becomes parallel branches.  Single stages (profiling) run
-// every group in order on the caller's stream.
-int launch_gemm(const scw_exec* ex, float* out, hipStream_t st, int stage) {
-    const int ng = (int)ex->groups.size();
-    if (stage != 0 || ng <= 1) {
-        for (const ZGroup& g : ex->groups) {
-            const int rc = launch_group(ex, g, out, st, stage);
-            if (rc != SCW_OK) return rc;
-        }
-        return SCW_OK;
-    }
-    SCW_HIP(hipEventRecord(ex->ev_fork, st));
-    for (int i = 0; i < ng; ++i) {
-        hipStream_t bs = ex->bstreams[i];
-        SCW_HIP(hipStreamWaitEvent(bs, ex->ev_fork, 0));
-        const int rc = launch_group(ex, ex->groups[i], out, bs, 0);
-        if (rc != SCW_OK) return rc;
-        SCW_HIP(hipEventRecord(ex->ev_join[i], bs));
-    }
-    for (int i = 0; i < ng; ++i) SCW_HIP(hipStreamWaitEvent(st, ex->ev_join[i], 0));
     return SCW_OK;
 }
 

@@ -109,17 +109,24 @@ struct CoefItem {
     int32_t pad;
 };
 
+// One wave-sized work item of the MFMA mode contraction (dense regions):
+// C_z[n] += sum_{c in [c0, c1)} Z[z][c] A[c][n] for real columns n in
+// [n0, n0 + 64) and the run z0 .. z0 + nz - 1 (nz <= 64).
+struct ModalItem {
+    int32_t region;
+    int32_t z0, nz;
+    int32_t n0;
+    int32_t c0, c1;
+    int32_t slot;
+    int32_t pad;
+};
+
 struct RegionDev {      // per-region data the GEMM-form kernels read (device)
-    int64_t ell_off;    // slot offset in ellA (3 float2 per slot)
-    int64_t ellc_off;   // slot offset in ellC (-1 when dense)
+    const float2* ellA; // the region's terms: [width][cells][3]
+    const int32_t* ellC;// kz index per term slot (nullptr when dense)
     int64_t x_off;      // float2 offset of X_r ([mx][nx])
     int32_t cells, width, mc, dense;
     int32_t mx, my, nslots, pad;
-};
-
-// A contiguous range of grid z and the coefficient items that serve it.
-struct ZGroup {
-    int z0, z1, i0, i1;
 };
 
 // Direct form helpers (scw_direct.hip).
@@ -149,15 +156,18 @@ struct scw_exec {
     int my_max = 0;
     int nslots_max = 1;
     std::vector<scw::CoefItem> items;
-    scw::DevBuf d_items, d_regions, d_zmeta, d_Z, d_X, d_C, d_U, d_Yfrag;
+    std::vector<scw::ModalItem> mitems;
+    scw::DevBuf d_items, d_mitems, d_regions, d_zmeta, d_Z, d_X, d_C, d_U, d_Yfrag;
+    int64_t tune_coef_items = 0;   // launch-shape choices (0 = default)
+    int64_t tune_main_blocks = 0;
+    std::vector<double> hx, hy;    // grid kept for re-planning launch shapes
+    std::vector<scw_plan*> batch;  // plans of a batch exec (empty for a single plan)
+    std::vector<double> hz;        // z of every output slice
+    std::vector<int> hzp;          // plan index (into batch, or 0) of every output slice
     int64_t c_elems = 0;      // float2 per slot of the C buffer
     int64_t ylo_delta = 0;    // element offset of the lo Y fragments (bf16 x3)
     int ysplit = 1;           // grid.z of the main GEMM
     int ytiles_per_split = 1;
-    std::vector<scw::ZGroup> groups;        // z groups of the GEMM form (>= 1)
-    std::vector<hipStream_t> bstreams;      // one branch stream per group when groups > 1
-    hipEvent_t ev_fork = nullptr;
-    std::vector<hipEvent_t> ev_join;
     scw::DevBuf d_out;        // staging for scw_exec_run_host
     // graph
     hipGraph_t graph = nullptr;

@@ -69,43 +69,61 @@ void DevBuf::release() {
 }
 
 // Group values that agree to within tol in both parts; returns distinct
-// representatives (first member of each group, in sorted order) and a group
-// index per input.
+// representatives (in sorted order) and a group index per input.  Exact
+// duplicates (the common case: producers compute kx, ky, kz with one formula
+// per order or mode) are merged by hashing first, so only the distinct values
+// are sorted.
+struct PairHash {
+    size_t operator()(const std::pair<uint64_t, uint64_t>& p) const {
+        return std::hash<uint64_t>()(p.first * 0x9E3779B97F4A7C15ull ^ p.second);
+    }
+};
+
 static void group_values(const std::vector<cd>& v, double tol, std::vector<cd>& reps, std::vector<int>& idx) {
     const size_t n = v.size();
-    std::vector<int> order(n);
-    for (size_t i = 0; i < n; ++i) order[i] = (int)i;
-    std::sort(order.begin(), order.end(), [&](int a, int b) {
-        if (v[a].real() != v[b].real()) return v[a].real() < v[b].real();
-        return v[a].imag() < v[b].imag();
+    std::unordered_map<std::pair<uint64_t, uint64_t>, int, PairHash> exact;
+    std::vector<cd> uniq;
+    std::vector<int> uid(n);
+    for (size_t i = 0; i < n; ++i) {
+        double re = v[i].real() + 0.0, im = v[i].imag() + 0.0;  // -0 == +0
+        uint64_t a, b;
+        std::memcpy(&a, &re, 8);
+        std::memcpy(&b, &im, 8);
+        auto it = exact.find({a, b});
+        if (it == exact.end()) {
+            it = exact.emplace(std::make_pair(a, b), (int)uniq.size()).first;
+            uniq.push_back(cd(re, im));
+        }
+        uid[i] = it->second;
+    }
+    const size_t u = uniq.size();
+    std::vector<int> order(u);
+    for (size_t i = 0; i < u; ++i) order[i] = (int)i;
+    std::sort(order.begin(), order.end(), [&](int x, int y) {
+        if (uniq[x].real() != uniq[y].real()) return uniq[x].real() < uniq[y].real();
+        return uniq[x].imag() < uniq[y].imag();
     });
     reps.clear();
-    idx.assign(n, -1);
-    for (size_t t = 0; t < n; ++t) {
-        const cd x = v[order[t]];
-        bool merged = false;
+    std::vector<int> grp(u, -1);
+    for (size_t t = 0; t < u; ++t) {
+        const cd x = uniq[order[t]];
+        int g = -1;
         // Look back over representatives whose real part is within tol.
-        for (int g = (int)reps.size() - 1; g >= 0; --g) {
-            if (x.real() - reps[g].real() > tol) break;
-            if (std::fabs(x.imag() - reps[g].imag()) <= tol) {
-                idx[order[t]] = g;
-                merged = true;
+        for (int k = (int)reps.size() - 1; k >= 0; --k) {
+            if (x.real() - reps[k].real() > tol) break;
+            if (std::fabs(x.imag() - reps[k].imag()) <= tol) {
+                g = k;
                 break;
             }
         }
-        if (!merged) {
+        if (g < 0) {
             reps.push_back(x);
-            idx[order[t]] = (int)reps.size() - 1;
+            g = (int)reps.size() - 1;
         }
+        grp[order[t]] = g;
     }
-}
-
-// exp(i*kz*z0 + ln s) * e  — E0 re-referenced without forming the two
-// factors separately (either may overflow alone).
-static inline cd reref(const cd e_hat, double lns, cd kz, double z0) {
-    const cd ph = cd(0, 1) * kz * z0;  // i kz z0
-    const double mag = std::exp(lns + ph.real());
-    return e_hat * cd(mag * std::cos(ph.imag()), mag * std::sin(ph.imag()));
+    idx.resize(n);
+    for (size_t i = 0; i < n; ++i) idx[i] = grp[uid[i]];
 }
 
 static const int MY_MAX = 64;   // largest ky count of the GEMM form (K = 2*my <= 128)
@@ -144,6 +162,15 @@ static void plan_region(RegionPlan& R, const scw_planewave64* W, const std::vect
     R.mc = (int)R.kz.size();
     const int64_t cells = (int64_t)R.mx * R.my;
 
+    // Per distinct kz: the re-referencing factor exp(i kz zref) as a real
+    // exponent and a unit rotation, so that every wave costs one real exp.
+    std::vector<double> zexp(R.mc);
+    std::vector<cd> zrot(R.mc);
+    for (int c = 0; c < R.mc; ++c) {
+        const cd ph = cd(0, 1) * R.kz[c] * R.zref;
+        zexp[c] = ph.real();
+        zrot[c] = cd(std::cos(ph.imag()), std::sin(ph.imag()));
+    }
     // Terms keyed by (cell, c); E0 summed in FP64 after re-referencing to zref.
     struct Term {
         int64_t cell;
@@ -178,12 +205,13 @@ static void plan_region(RegionPlan& R, const scw_planewave64* W, const std::vect
                 t = it->second;
             }
         }
-        const cd kzc = R.kz[ic[i]];
-        for (int c = 0; c < 3; ++c) terms[(size_t)t].e[c] += reref(ehat[3 * g + c], lns[g], kzc, R.zref);
+        // E0 exp(i kz zref) = (E0/s) exp(ln s - Im(kz) zref) (cos, sin)(Re(kz) zref)
+        const double mag = std::exp(lns[g] + zexp[ic[i]]);
+        const cd f = zrot[ic[i]] * mag;
+        for (int c = 0; c < 3; ++c) terms[(size_t)t].e[c] += ehat[3 * g + c] * f;
     }
-    // Order terms by (cell, c) so ELL columns hold ascending kz indices.
-    std::sort(terms.begin(), terms.end(),
-              [](const Term& x, const Term& y) { return x.cell != y.cell ? x.cell < y.cell : x.c < y.c; });
+    // Terms keep their first-appearance order: dense regions index the ELL by
+    // kz index, sparse ones fill each cell's columns in that order.
     R.n_terms = (int64_t)terms.size();
     std::vector<int> count(cells, 0);
     for (auto& t : terms) count[t.cell]++;

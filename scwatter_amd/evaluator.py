@@ -117,6 +117,10 @@ class Exec:
         check(_native.lib().scw_exec_run_host(self.handle, out.ctypes.data, stream))
         return out
 
+    def set_launch(self, coef_items=0, main_blocks=0):
+        """GEMM-form launch shape (0 = default); rebuilds the work lists."""
+        check(_native.lib().scw_exec_set_launch(self.handle, int(coef_items), int(main_blocks)))
+
     def graph_capture(self, out_ptr, stream=None):
         check(_native.lib().scw_exec_graph_capture(self.handle, out_ptr, stream))
 
@@ -133,6 +137,34 @@ class Exec:
             self.destroy()
         except Exception:
             pass
+
+
+class BatchExec(Exec):
+    """Independent structures in one GEMM-form run: instance i is plans[i] on
+    xs x ys x zs_list[i]; output complex64 (sum nz_i, ny, nx, 3), instances
+    concatenated along z."""
+
+    def __init__(self, plans, xs, ys, zs_list, region_tol=0.0, dtype="bf16x3"):
+        self.plans = list(plans)
+        self.xs = np.ascontiguousarray(np.atleast_1d(xs), np.float64)
+        self.ys = np.ascontiguousarray(np.atleast_1d(ys), np.float64)
+        self.zs_list = [np.ascontiguousarray(np.atleast_1d(z), np.float64) for z in zs_list]
+        n = len(self.plans)
+        handles = (_vp * n)(*[p.handle for p in self.plans])
+        zptrs = (ctypes.POINTER(ctypes.c_double) * n)(*[_dptr(z) for z in self.zs_list])
+        nzs = (ctypes.c_size_t * n)(*[z.size for z in self.zs_list])
+        h = _vp()
+        check(_native.lib().scw_batch_exec_create(n, handles, _dptr(self.xs), self.xs.size, _dptr(self.ys),
+                                                  self.ys.size, zptrs, nzs, float(region_tol),
+                                                  _native.DTYPES[dtype], ctypes.byref(h)))
+        self.handle = h.value
+        self.zs = np.concatenate(self.zs_list)
+        self.shape = (self.zs.size, self.ys.size, self.xs.size, 3)
+        self.offsets = np.concatenate([[0], np.cumsum([z.size for z in self.zs_list])])
+
+    def instance(self, E, i):
+        """Slice instance i's field out of a batch output."""
+        return E[self.offsets[i]:self.offsets[i + 1]]
 
 
 # ---- coordinate rules of the reference ---------------------------------------

@@ -241,19 +241,33 @@ def test_errors_are_reported():
         Plan(WaveTables(W, np.zeros(0), np.array([0], np.int32), np.array([n], np.int32)), device=99)
 
 
-def test_forked_z_groups_match_single_group(tile_small, monkeypatch):
-    """The fork/join z-group path (branch streams, also inside a captured graph) is bit-identical."""
-    from scwatter_amd import hip
-    plan, g, _ = tile_small
-    one = Exec(plan, *g, form="gemm", dtype="bf16x3").run_host()
-    monkeypatch.setenv("SCW_ZGROUPS", "3")
-    ex = Exec(plan, *g, form="gemm", dtype="bf16x3")
-    assert np.array_equal(ex.run_host(), one)
+@pytest.mark.parametrize("dtype", ["bf16x3", "fp32"])
+def test_batch_of_uneven_instances(dtype):
+    """One batch run over instances of different order counts equals each instance against the oracle."""
+    from scwatter_amd import BatchExec, hip
+    specs = [(3, 11, 0.6, 5.0), (5, 12, 0.9, 20.0), (7, 9, 0.45, 35.0)]
+    xs, ys = configs.axis(-6, 6, 20), configs.axis(-6, 6, 16)
+    plans, tables, zs_list = [], [], []
+    for M, nz, lam, ang in specs:
+        t = configs.tile_sample_structure(M=M, Z=1, size=(12.0, 12.0, 2.0), lam=lam, seed=M,
+                                          incidence_deg=ang).unpack()
+        tables.append(t)
+        plans.append(Plan(t))
+        zs_list.append(configs.axis(-0.9, 0.9, nz))
+    bx = BatchExec(plans, xs, ys, zs_list, dtype=dtype)
+    assert bx.info()["form"] == "gemm"
+    E = bx.run_host()
+    for i, (t, zs) in enumerate(zip(tables, zs_list)):
+        ref = oracle.eval_grid_f64(t.W, t.z_layers, t.begin, t.end, xs, ys, zs)
+        _check(bx.instance(E, i), ref, TOL[dtype], f"batch {dtype} instance {i}")
+    # a different launch shape gives the same field
+    bx.set_launch(coef_items=512, main_blocks=64)
+    E2 = bx.run_host()
+    assert oracle.rel_l2(E2, E) < 1e-6
     s = hip.Stream()
-    buf = hip.DeviceArray(ex.shape, np.complex64)
-    ex.graph_capture(buf.ptr, s.handle)
-    for _ in range(2):
-        ex.graph_launch(s.handle)
-        s.synchronize()
-        assert np.array_equal(buf.to_host(), one)
+    buf = hip.DeviceArray(bx.shape, np.complex64)
+    bx.graph_capture(buf.ptr, s.handle)
+    bx.graph_launch(s.handle)
+    s.synchronize()
+    assert np.array_equal(buf.to_host(), E2)
     s.destroy()
