@@ -30,6 +30,7 @@ namespace scw {
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+typedef uint32_t uint32x4 __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
 // C_z[a,b,comp] partial sums over a term slice, for up to 8 z of one region.
@@ -219,30 +220,62 @@ __device__ inline float bp_value(const float* __restrict__ Ub, int64_t ncols, in
 // written once and not read back, so the stores are non-temporal.
 constexpr int TROW = 256 + 4;
 
+// One 16-byte (VEC) or 4 x 4-byte store per lane into row `orow` through a
+// raw buffer resource whose range is the row (empty when the row is past ny):
+// the hardware range check drops the lanes and rows outside the field, so
+// the stores carry no branches and the wait counters see the same number of
+// stores on every path.
+constexpr int kBufferRsrcWord3 = 0x00020000;  // raw dword buffer, gfx9 family
+constexpr int kStoreNT = 2;                   // non-temporal
+
+template <bool VEC>
+__device__ inline void row_store(float* orow, bool row_ok, int64_t n, int64_t ncols, const floatx4& v) {
+    const uint32_t nrec = row_ok ? (uint32_t)(ncols * 4) : 0u;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(orow, 0, (int)nrec, kBufferRsrcWord3);
+    const int off = (int)(n * 4);
+    if (VEC) {
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(uint32x4, v), rs, off, 0, kStoreNT);
+    } else {
+        // Each element gets its own offset, moved far out of range past the
+        // row's end, so a partly valid group of 4 columns is written element
+        // by element and the stores are never merged into one 16-byte access.
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const float f = v[r];
+            const int off_r = n + r < ncols ? off + 4 * r : 0x7ffffff0;
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, f), rs, off_r, 0, kStoreNT);
+        }
+    }
+}
+
 template <bool VEC>
 __device__ inline void tile_out(float (*T)[TROW], const floatx4 (&acc)[4], float* __restrict__ oz, int64_t ncols,
                                 int ny, int yt, int64_t nwg0) {
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
 #pragma unroll
     for (int it = 0; it < 4; ++it) *reinterpret_cast<floatx4*>(&T[li][64 * wv + 16 * it + 4 * lk]) = acc[it];
     __syncthreads();
+    floatx4 v[4];
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) v[rr] = *reinterpret_cast<const floatx4*>(&T[4 * rr + wv][4 * lane]);
     const int64_t n = nwg0 + 4 * lane;
+    const int y0 = yt * 16 + wv;
 #pragma unroll
-    for (int rr = 0; rr < 4; ++rr) {
-        const int row = 4 * rr + wv;
-        const int y = yt * 16 + row;
-        if (y < ny) {
-            const floatx4 v = *reinterpret_cast<const floatx4*>(&T[row][4 * lane]);
-            float* orow = oz + (int64_t)y * ncols;
-            if (VEC) {
-                if (n < ncols) __builtin_nontemporal_store(v, reinterpret_cast<floatx4*>(orow + n));
-            } else {
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    if (n + r < ncols) __builtin_nontemporal_store(v[r], orow + n + r);
-            }
-        }
-    }
+    for (int rr = 0; rr < 4; ++rr)
+        row_store<VEC>(oz + (int64_t)(y0 + 4 * rr) * ncols, y0 + 4 * rr < ny, n, ncols, v[rr]);
+}
+
+// Slice of a region with no waves: the workgroup's columns of y tiles
+// [yt0, yt1) are zero.
+template <bool VEC>
+__device__ inline void zero_tiles(float* __restrict__ oz, int64_t ncols, int ny, int yt0, int yt1, int64_t nwg0) {
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int64_t n = nwg0 + 4 * lane;
+    const int yend = min(ny, 16 * yt1);
+    for (int y = 16 * yt0 + wv; y < yend; y += 4)
+        row_store<VEC>(oz + (int64_t)y * ncols, true, n, ncols, floatx4{0.0f, 0.0f, 0.0f, 0.0f});
 }
 
 // ---------------------------------------------------------------------------
@@ -250,7 +283,7 @@ __device__ inline void tile_out(float (*T)[TROW], const floatx4 (&acc)[4], float
 // Workgroup = 4 waves, each owns 64 output columns (n) of one z-slice and
 // walks y in tiles of 16; its A-operand fragment (from U) stays in registers.
 // The B-operand (Y) fragment of a tile is four k-steps per 16-byte load,
-// prefetched one tile ahead.
+// prefetched two tiles ahead.
 // ---------------------------------------------------------------------------
 template <int KMAX, bool VEC>
 __global__ __launch_bounds__(256) void main_f32(const ZMeta* __restrict__ zmeta, const float2* __restrict__ U,
@@ -265,6 +298,14 @@ __global__ __launch_bounds__(256) void main_f32(const ZMeta* __restrict__ zmeta,
     const int64_t n0 = (int64_t)blockIdx.x * 256 + wv * 64;  // waves past ncols compute zeros
     const int li = lane & 15, lk = lane >> 4;
     const float* Ub = reinterpret_cast<const float*>(U + zm.u_off);
+    const int ytiles = (ny + 15) >> 4;
+    const int yt0 = blockIdx.z * ytiles_per_split;
+    const int yt1 = min(ytiles, yt0 + ytiles_per_split);
+    float* oz = out + (int64_t)z * ny * ncols;
+    if (K16 == 0) {
+        zero_tiles<VEC>(oz, ncols, ny, yt0, yt1, (int64_t)blockIdx.x * 256);
+        return;
+    }
     __shared__ float T[2][16][TROW];
 
     float uf[4][KMAX / 4];
@@ -278,35 +319,49 @@ __global__ __launch_bounds__(256) void main_f32(const ZMeta* __restrict__ zmeta,
         }
     }
 
-    const int ytiles = (ny + 15) >> 4;
-    const int yt0 = blockIdx.z * ytiles_per_split;
-    const int yt1 = min(ytiles, yt0 + ytiles_per_split);
     const floatx4* Yf = reinterpret_cast<const floatx4*>(Yfrag + zm.yfrag_off) + lane;
-    float* oz = out + (int64_t)z * ny * ncols;
-    floatx4 ycur[Q], ynext[Q];
+    // Y fragment loads are unconditional (k-slices past the region's K16
+    // re-read its last one and are not used; prefetches past the last tile
+    // re-read that tile), so every path issues the same loads.  The
+    // loop is unrolled by two over named register sets, so a prefetch is
+    // waited for only when its tile is computed, after the previous tile's
+    // stores were issued.
+    auto yload = [&](int yt, floatx4 (&y)[Q]) {
 #pragma unroll
-    for (int q = 0; q < Q; ++q) ycur[q] = (q < K16 && yt0 < yt1) ? Yf[((int64_t)yt0 * K16 + q) * 64] : floatx4{0, 0, 0, 0};
-    for (int yt = yt0; yt < yt1; ++yt) {
-#pragma unroll
-        for (int q = 0; q < Q; ++q)
-            ynext[q] = (q < K16 && yt + 1 < yt1) ? Yf[((int64_t)(yt + 1) * K16 + q) * 64] : floatx4{0, 0, 0, 0};
+        for (int q = 0; q < Q; ++q) y[q] = Yf[((int64_t)yt * K16 + min(q, K16 - 1)) * 64];
+    };
+    auto tile = [&](int yt, const floatx4 (&y)[Q]) {
         floatx4 acc[4];
 #pragma unroll
         for (int it = 0; it < 4; ++it) acc[it] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            if (q < K16) {
+        for (int q = 0; q < Q; ++q)
+            if (q < K16)  // uniform; k-slices past the region's contribute zero
 #pragma unroll
                 for (int e = 0; e < 4; ++e)
 #pragma unroll
                     for (int it = 0; it < 4; ++it)
-                        acc[it] = __builtin_amdgcn_mfma_f32_16x16x4f32(uf[it][4 * q + e], ycur[q][e], acc[it], 0, 0, 0);
-            }
-        }
+                        acc[it] = __builtin_amdgcn_mfma_f32_16x16x4f32(uf[it][4 * q + e], y[q][e], acc[it], 0, 0, 0);
         tile_out<VEC>(T[yt & 1], acc, oz, ncols, ny, yt, (int64_t)blockIdx.x * 256);
-#pragma unroll
-        for (int q = 0; q < Q; ++q) ycur[q] = ynext[q];
+    };
+    // The first pair of tiles is peeled so that the loop is entered with the
+    // same loads and stores in flight as at its back edge.
+    floatx4 y0[Q], y1[Q];
+    yload(yt0, y0);
+    yload(min(yt0 + 1, yt1 - 1), y1);
+    tile(yt0, y0);
+    yload(min(yt0 + 2, yt1 - 1), y0);
+    if (yt0 + 1 >= yt1) return;
+    tile(yt0 + 1, y1);
+    yload(min(yt0 + 3, yt1 - 1), y1);
+    int yt = yt0 + 2;
+    for (; yt + 1 < yt1; yt += 2) {
+        tile(yt, y0);
+        yload(min(yt + 2, yt1 - 1), y0);
+        tile(yt + 1, y1);
+        yload(min(yt + 3, yt1 - 1), y1);
     }
+    if (yt < yt1) tile(yt, y0);
 }
 
 // ---------------------------------------------------------------------------
@@ -357,6 +412,14 @@ __global__ __launch_bounds__(256) void main_16(const ZMeta* __restrict__ zmeta, 
     const int64_t n0 = (int64_t)blockIdx.x * 256 + wv * 64;  // waves past ncols compute zeros
     const int li = lane & 15, lk = lane >> 4;
     const float* Ub = reinterpret_cast<const float*>(U + zm.u_off);
+    const int ytiles = (ny + 15) >> 4;
+    const int yt0 = blockIdx.z * ytiles_per_split;
+    const int yt1 = min(ytiles, yt0 + ytiles_per_split);
+    float* oz = out + (int64_t)z * ny * ncols;
+    if (K32 == 0) {
+        zero_tiles<VEC>(oz, ncols, ny, yt0, yt1, (int64_t)blockIdx.x * 256);
+        return;
+    }
     __shared__ float T[2][16][TROW];
 
     vec ahi[4][S], alo[4][S];
@@ -376,48 +439,50 @@ __global__ __launch_bounds__(256) void main_16(const ZMeta* __restrict__ zmeta, 
         }
     }
 
-    const int ytiles = (ny + 15) >> 4;
-    const int yt0 = blockIdx.z * ytiles_per_split;
-    const int yt1 = min(ytiles, yt0 + ytiles_per_split);
     const uint16_t* Yf = Yfrag + zm.yfrag_off + (int64_t)lane * 8;
-    float* oz = out + (int64_t)z * ny * ncols;
-    // Y fragments of the current tile and the next one (prefetched).
-    vec yh[S], yl[S], nh[S], nl[S];
+    // Unconditional Y loads and a loop unrolled by two (see main_f32).
     auto yload = [&](int yt, vec (&h)[S], vec (&l)[S]) {
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            const bool ok = s < K32 && yt < yt1;
-            const uint16_t* p = Yf + ((int64_t)yt * K32 + s) * 512;
-            h[s] = ok ? *reinterpret_cast<const vec*>(p) : vec{};
-            if constexpr (MODE == 1) l[s] = ok ? *reinterpret_cast<const vec*>(p + ylo_delta) : vec{};
+            const uint16_t* p = Yf + ((int64_t)yt * K32 + min(s, K32 - 1)) * 512;
+            h[s] = *reinterpret_cast<const vec*>(p);
+            if constexpr (MODE == 1) l[s] = *reinterpret_cast<const vec*>(p + ylo_delta);
         }
     };
-    yload(yt0, yh, yl);
-    for (int yt = yt0; yt < yt1; ++yt) {
-        yload(yt + 1, nh, nl);
+    auto tile = [&](int yt, const vec (&h)[S], const vec (&l)[S]) {
         floatx4 acc[4];
 #pragma unroll
         for (int it = 0; it < 4; ++it) acc[it] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            if (s < K32) {
-                if constexpr (MODE == 1) {
+            if (s >= K32) continue;  // uniform; k-slices past the region's contribute zero
+            if constexpr (MODE == 1) {
 #pragma unroll
-                    for (int it = 0; it < 4; ++it) acc[it] = mfma16<MODE>(alo[it][s], yh[s], acc[it]);
+                for (int it = 0; it < 4; ++it) acc[it] = mfma16<MODE>(alo[it][s], h[s], acc[it]);
 #pragma unroll
-                    for (int it = 0; it < 4; ++it) acc[it] = mfma16<MODE>(ahi[it][s], yl[s], acc[it]);
-                }
-#pragma unroll
-                for (int it = 0; it < 4; ++it) acc[it] = mfma16<MODE>(ahi[it][s], yh[s], acc[it]);
+                for (int it = 0; it < 4; ++it) acc[it] = mfma16<MODE>(ahi[it][s], l[s], acc[it]);
             }
+#pragma unroll
+            for (int it = 0; it < 4; ++it) acc[it] = mfma16<MODE>(ahi[it][s], h[s], acc[it]);
         }
         tile_out<VEC>(T[yt & 1], acc, oz, ncols, ny, yt, (int64_t)blockIdx.x * 256);
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            yh[s] = nh[s];
-            if constexpr (MODE == 1) yl[s] = nl[s];
-        }
+    };
+    vec h0[S], l0[S], h1[S], l1[S];
+    yload(yt0, h0, l0);
+    yload(min(yt0 + 1, yt1 - 1), h1, l1);
+    tile(yt0, h0, l0);
+    yload(min(yt0 + 2, yt1 - 1), h0, l0);
+    if (yt0 + 1 >= yt1) return;
+    tile(yt0 + 1, h1, l1);
+    yload(min(yt0 + 3, yt1 - 1), h1, l1);
+    int yt = yt0 + 2;
+    for (; yt + 1 < yt1; yt += 2) {
+        tile(yt, h0, l0);
+        yload(min(yt + 2, yt1 - 1), h0, l0);
+        tile(yt + 1, h1, l1);
+        yload(min(yt + 3, yt1 - 1), h1, l1);
     }
+    if (yt < yt1) tile(yt, h0, l0);
 }
 
 template <int KMAX, bool VEC>

@@ -113,6 +113,20 @@ def test_tile_sample_gemm(tile_small, dtype):
     _check(Exec(plan, *g, form="gemm", dtype=dtype).run_host(), ref, TOL[dtype], f"tile sample gemm {dtype}")
 
 
+@pytest.mark.parametrize("dtype", ["fp32", "bf16x3", "bf16", "fp16"])
+@pytest.mark.parametrize("nx,ny", [(1, 33), (3, 17), (45, 5), (50, 19), (67, 1)])
+def test_gemm_ragged_grids(tile_small, dtype, nx, ny):
+    """Grids whose rows are not whole 16-byte groups (odd nx: 6 nx floats per
+    row), whose last 256-column block is partial (nx = 50, 67) and whose y
+    extent is not a whole number of 16-row tiles."""
+    plan, _, _ = tile_small
+    xs, ys, zs = configs.axis(-7, 7.5, nx), configs.axis(-6, 8, ny), configs.axis(-3, 3, 9)
+    t = plan.tables
+    ref = oracle.eval_grid_f64(t.W, t.z_layers, t.begin, t.end, xs, ys, zs)
+    _check(Exec(plan, xs, ys, zs, form="gemm", dtype=dtype).run_host(), ref, TOL[dtype],
+           f"ragged grid {nx}x{ny}x{zs.size} gemm {dtype}")
+
+
 @pytest.fixture(scope="module")
 def stack_small():
     rng = np.random.default_rng(11)
