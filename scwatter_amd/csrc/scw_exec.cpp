@@ -216,38 +216,55 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
         ex->kmax = kmax <= 32 ? 32 : kmax <= 64 ? 64 : kmax <= 96 ? 96 : 128;
 
     // Runs: consecutive z of one virtual region.  Dense eigenmode regions
-    // contract on MFMA when the dtype is 16-bit (runs of up to 64 z, every
-    // term read once per run); other regions use the VALU ELL contraction
-    // (runs of up to 8 z).
+    // (every cell holds every kz, term index = kz index) contract on MFMA in
+    // runs of up to 64 z, every term read once per run; other regions use the
+    // VALU ELL contraction in runs of up to 8 z.  A maximal stretch longer
+    // than 64 z becomes several runs of one group, whose items interleave so
+    // that the runs of a stretch read the same terms at the same time.
     struct Run {
         int z0, z1, v;
         bool modal;
+        int group;
     };
     std::vector<Run> runs;
+    // Shortest stretch contracted on MFMA: every stretch for the 16-bit
+    // dtypes; for FP32 (MFMA at the VALU's FP32 rate) only stretches the VALU
+    // path would have to read more than once.
+    const int64_t modal_min = tunable("SCW_MODAL_MIN_RUN", is16 ? 1 : 9);
+    int ngroups = 0;
     for (int z = 0; z < nz;) {
         const int v = zv[z];
         int zend = z + 1;  // maximal stretch of consecutive z in region v
         while (zend < nz && zv[zend] == v) ++zend;
-        // The MFMA contraction pays off once the VALU path (8 z per pass)
-        // would read the terms more than once.
-        const bool modal = is16 && rd[v].dense && rd[v].width > 0 && zend - z > 8;
+        const bool modal = rd[v].dense && rd[v].width > 0 && zend - z >= modal_min;
         const int cap = modal ? 64 : 8;
-        for (int z0 = z; z0 < zend; z0 += cap) runs.push_back({z0, std::min(zend, z0 + cap), v, modal});
+        const int nrun = (zend - z + cap - 1) / cap;  // balanced runs
+        for (int k = 0; k < nrun; ++k)
+            runs.push_back({z + (int)((int64_t)(zend - z) * k / nrun), z + (int)((int64_t)(zend - z) * (k + 1) / nrun),
+                            v, modal, ngroups});
+        ++ngroups;
         z = zend;
     }
     // Slices of the term (or mode) range: enough wave items to cover every
     // SIMD several times, with at least 16 terms per slice.
+    std::vector<int> mper(NV, 16);  // mode-slice width of dense regions
     {
         int64_t base = 0;
         for (const Run& rn : runs) {
             const RegionDev& d = rd[rn.v];
-            base += rn.modal ? (6LL * d.cells + 63) / 64 : (d.cells + 63) / 64;
+            base += rn.modal ? (3LL * d.cells + 63) / 64 : (d.cells + 63) / 64;
         }
         const int64_t target = ex->tune_coef_items > 0 ? ex->tune_coef_items : tunable("SCW_COEF_ITEMS", 4096);
         const int want = base > 0 ? (int)std::min<int64_t>(64, (target + base - 1) / base) : 1;
         for (int v = 0; v < NV; ++v) {
-            rd[v].nslots = std::max(1, std::min(want, rd[v].width / 16));
-            ex->nslots_max = std::max(ex->nslots_max, rd[v].nslots);
+            int n = std::max(1, std::min(want, rd[v].width / 16));
+            if (rd[v].dense && rd[v].width > 0) {
+                // mode slices are whole 16-mode steps; every slot gets modes
+                mper[v] = round_up((rd[v].width + n - 1) / n, 16);
+                n = (rd[v].width + mper[v] - 1) / mper[v];
+            }
+            rd[v].nslots = n;
+            ex->nslots_max = std::max(ex->nslots_max, n);
         }
     }
 
@@ -293,41 +310,130 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
     // Work items.
     ex->items.clear();
     ex->mitems.clear();
-    for (const Run& rn : runs) {
-        const RegionDev& d = rd[rn.v];
-        if (d.cells == 0 || d.width == 0) continue;
-        b_coef += 24.0 * (double)d.cells * d.width + 8.0 * d.mc * (rn.z1 - rn.z0);  // terms once per run, Z rows
-        if (rn.modal) {
-            const int per = round_up((d.width + d.nslots - 1) / d.nslots, 16);
-            for (int n0 = 0; n0 < 6 * d.cells; n0 += 64)
-                for (int s2 = 0; s2 < d.nslots; ++s2) {
-                    ModalItem it;
-                    it.region = rn.v;
-                    it.z0 = rn.z0;
-                    it.nz = rn.z1 - rn.z0;
-                    it.n0 = n0;
-                    it.c0 = std::min(d.width, s2 * per);
-                    it.c1 = std::min(d.width, (s2 + 1) * per);
-                    it.slot = s2;
-                    it.pad = 0;
-                    ex->mitems.push_back(it);
+    std::vector<uint32_t> zf;  // phase fragments of the modal runs
+    const bool f32 = ex->dtype == SCW_DTYPE_FP32;
+    for (size_t r0 = 0; r0 < runs.size();) {
+        size_t r1 = r0 + 1;  // runs of one group: [r0, r1)
+        while (r1 < runs.size() && runs[r1].group == runs[r0].group) ++r1;
+        const RegionDev& d = rd[runs[r0].v];
+        if (d.cells == 0 || d.width == 0) {
+            r0 = r1;
+            continue;
+        }
+        for (size_t r = r0; r < r1; ++r)
+            b_coef += 24.0 * (double)d.cells * d.width + 8.0 * d.mc * (runs[r].z1 - runs[r].z0);
+        if (runs[r0].modal) {
+            const RegionPlan& R = vreg[runs[r0].v].first->regions[vreg[runs[r0].v].second];
+            const int T = (d.width + 15) / 16;
+            std::vector<ModalItem> proto(r1 - r0);
+            for (size_t r = r0; r < r1; ++r) {
+                const Run& rn = runs[r];
+                const int nzr = rn.z1 - rn.z0;
+                const int MT = nzr <= 8 ? 1 : nzr <= 16 ? 2 : nzr <= 32 ? 4 : 8;
+                ModalItem& it = proto[r - r0];
+                it.region = rn.v;
+                it.z0 = rn.z0;
+                it.nz = nzr;
+                it.T = T;
+                it.zf_off = (int64_t)zf.size();
+                // Z'[i][k] with rows i = (z, re|im), k = (mode c, re|im):
+                //   (re, re) = Re Z, (re, im) = -Im Z, (im, re) = Im Z, (im, im) = Re Z,
+                // Z = exp(i kz_c (z - zref)).  Per (mt, t, lane) 8 words: FP32
+                // values for MFMA q = 0..7 (c = 16t + g + 4(q/2), part q%2), or
+                // 8 high then 8 low 16-bit halves (c = 16t + 4g + e/2, part e%2),
+                // g = lane / 16, row i = lane % 16.
+                std::vector<cd> zc((size_t)nzr * d.width);
+                for (int zi = 0; zi < nzr; ++zi)
+                    for (int c = 0; c < d.width; ++c) {
+                        const cd val = std::exp(cd(0, 1) * R.kz[c] * (ze[rn.z0 + zi].z - R.zref));
+                        if (!region_ok(val))
+                            return fail(SCW_ERR_UNSUPPORTED, "region %d: exp(i kz (z - zref)) outside FP32 range at z=%g",
+                                        vreg[rn.v].second, ze[rn.z0 + zi].z);
+                        zc[(size_t)zi * d.width + c] = val;
+                    }
+                zf.resize(zf.size() + (size_t)MT * T * 64 * 8);
+                uint32_t* out = zf.data() + it.zf_off;
+                for (int mt = 0; mt < MT; ++mt)
+                    for (int t = 0; t < T; ++t)
+                        for (int l = 0; l < 64; ++l) {
+                            const int i = l & 15, g = l >> 4, zi = 8 * mt + (i >> 1), o = i & 1;
+                            double v8[8];
+                            for (int w = 0; w < 8; ++w) {
+                                const int c = f32 ? 16 * t + g + 4 * (w >> 1) : 16 * t + 4 * g + (w >> 1);
+                                const int part = w & 1;
+                                double val = 0.0;
+                                if (zi < nzr && c < d.width) {
+                                    const cd zv2 = zc[(size_t)zi * d.width + c];
+                                    val = o == 0 ? (part == 0 ? zv2.real() : -zv2.imag())
+                                                 : (part == 0 ? zv2.imag() : zv2.real());
+                                }
+                                v8[w] = val;
+                            }
+                            uint32_t* wd = out + (((int64_t)mt * T + t) * 64 + l) * 8;
+                            if (f32) {
+                                for (int w = 0; w < 8; ++w) {
+                                    const float fv = (float)v8[w];
+                                    std::memcpy(&wd[w], &fv, 4);
+                                }
+                            } else {
+                                uint16_t hi[8], lo[8];
+                                for (int w = 0; w < 8; ++w) {
+                                    if (ex->dtype == SCW_DTYPE_FP16) {
+                                        hi[w] = f16_rne((float)v8[w]);
+                                        lo[w] = 0;
+                                    } else {
+                                        hi[w] = bf16_rne((float)v8[w]);
+                                        lo[w] = bf16_rne((float)(v8[w] - (double)bf16_f32(hi[w])));
+                                    }
+                                }
+                                for (int w = 0; w < 4; ++w) {
+                                    wd[w] = (uint32_t)hi[2 * w] | ((uint32_t)hi[2 * w + 1] << 16);
+                                    wd[4 + w] = (uint32_t)lo[2 * w] | ((uint32_t)lo[2 * w + 1] << 16);
+                                }
+                            }
+                        }
+            }
+            // Slot-major: the waves of a workgroup share a mode slice, so they
+            // stream the same phase fragments at the same time.
+            const int per = mper[runs[r0].v];
+            for (int s2 = 0; s2 < d.nslots; ++s2)
+                for (int m0 = 0; m0 < 3 * d.cells; m0 += 64) {
+                    const int c0 = s2 * per, c1 = std::min(16 * T, (s2 + 1) * per);  // c0 < width
+                    for (ModalItem it : proto) {
+                        it.m0 = m0;
+                        it.c0 = c0;
+                        it.c1 = c1;
+                        it.slot = s2;
+                        ex->mitems.push_back(it);
+                    }
                 }
         } else {
             const int per = (d.width + d.nslots - 1) / d.nslots;
-            for (int c0 = 0; c0 < d.cells; c0 += 64)
-                for (int s2 = 0; s2 < d.nslots; ++s2) {
-                    CoefItem it;
-                    it.region = rn.v;
-                    it.cell0 = c0;
-                    it.j0 = std::min(d.width, s2 * per);
-                    it.j1 = std::min(d.width, (s2 + 1) * per);
-                    it.z0 = rn.z0;
-                    it.nz = rn.z1 - rn.z0;
-                    it.slot = s2;
-                    it.pad = 0;
-                    ex->items.push_back(it);
-                }
+            for (size_t r = r0; r < r1; ++r)
+                for (int c0 = 0; c0 < d.cells; c0 += 64)
+                    for (int s2 = 0; s2 < d.nslots; ++s2) {
+                        CoefItem it;
+                        it.region = runs[r].v;
+                        it.cell0 = c0;
+                        it.j0 = std::min(d.width, s2 * per);
+                        it.j1 = std::min(d.width, (s2 + 1) * per);
+                        it.z0 = runs[r].z0;
+                        it.nz = runs[r].z1 - runs[r].z0;
+                        it.slot = s2;
+                        it.pad = 0;
+                        ex->items.push_back(it);
+                    }
         }
+        r0 = r1;
+    }
+    // Modal items grouped by row-tile class, one launch per class.
+    {
+        auto cls = [](const ModalItem& it) { return it.nz <= 8 ? 0 : it.nz <= 16 ? 1 : it.nz <= 32 ? 2 : 3; };
+        std::stable_sort(ex->mitems.begin(), ex->mitems.end(),
+                         [&](const ModalItem& a, const ModalItem& b) { return cls(a) < cls(b); });
+        for (int k = 0; k <= 4; ++k) ex->mclass_off[k] = 0;
+        for (const ModalItem& it : ex->mitems) ++ex->mclass_off[cls(it) + 1];
+        for (int k = 1; k <= 4; ++k) ex->mclass_off[k] += ex->mclass_off[k - 1];
     }
     ex->stage_flops[1] = f_coef;
     ex->stage_flops[2] = f_u;
@@ -337,7 +443,7 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
     ex->stage_bytes[3] = b_main;
     ex->stage_flops[0] = ex->flops = f_coef + f_u + f_main;
     ex->stage_bytes[0] = ex->bytes = b_coef + b_u + b_main;
-    ex->launches = (ex->items.empty() ? 0 : 1) + (ex->mitems.empty() ? 0 : 1) + (ex->my_max > 0 ? 1 : 0) + 1;
+    ex->launches = (ex->items.empty() ? 0 : 1) + (int)((ex->mclass_off[1] > ex->mclass_off[0]) + (ex->mclass_off[2] > ex->mclass_off[1]) + (ex->mclass_off[3] > ex->mclass_off[2]) + (ex->mclass_off[4] > ex->mclass_off[3])) + (ex->my_max > 0 ? 1 : 0) + 1;
 
     const int gx = (int)((6LL * nx + 255) / 256);
     // Each wave keeps its A-operand fragment (from U_z) in registers for all
@@ -355,6 +461,7 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
     if ((st = ex->d_items.upload(ex->items.data(), ex->items.size() * sizeof(CoefItem))) != SCW_OK) return st;
     if ((st = ex->d_mitems.upload(ex->mitems.data(), ex->mitems.size() * sizeof(ModalItem))) != SCW_OK) return st;
     if ((st = ex->d_Z.upload(Z.data(), Z.size() * sizeof(float))) != SCW_OK) return st;
+    if ((st = ex->d_Zf.upload(zf.data(), zf.size() * sizeof(uint32_t))) != SCW_OK) return st;
     if ((st = ex->d_X.upload(X.data(), X.size() * sizeof(float))) != SCW_OK) return st;
     if ((st = ex->d_C.alloc((size_t)ex->nslots_max * std::max<int64_t>(c_off, 1) * 8)) != SCW_OK) return st;
     if ((st = ex->d_U.alloc((size_t)std::max<int64_t>(u_off, 1) * 8)) != SCW_OK) return st;
@@ -387,7 +494,7 @@ static void exec_free(scw_exec* ex) {
     if (ex->graph) (void)hipGraphDestroy(ex->graph);
     if (ex->graph_stream) (void)hipStreamDestroy(ex->graph_stream);
     for (DevBuf* b : {&ex->d_dx, &ex->d_dy, &ex->d_z, &ex->d_zslab, &ex->d_items, &ex->d_mitems, &ex->d_regions, &ex->d_zmeta,
-                      &ex->d_Z, &ex->d_X, &ex->d_C, &ex->d_U, &ex->d_Yfrag, &ex->d_out})
+                      &ex->d_Z, &ex->d_Zf, &ex->d_X, &ex->d_C, &ex->d_U, &ex->d_Yfrag, &ex->d_out})
         b->release();
     delete ex;
 }

@@ -56,18 +56,18 @@ template <int NZ, bool DENSE>
 __device__ inline void coef_loop(float (&acc)[8][6], const float2* __restrict__ Ab,
                                  const int32_t* __restrict__ Cb, int64_t stride, const float2* __restrict__ Z,
                                  const int64_t (&zrow)[8], int j0, int j1) {
-    int j = j0;
-    for (; j + 4 <= j1; j += 4) {
-        float2 a[4][3];
-        int c[4];
+    typedef float2 Group[4][3];
+    auto load = [&](Group& a, int (&c)[4], int jj) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const float2* p = Ab + (int64_t)(j + u) * stride * 3;
+            const float2* p = Ab + (int64_t)(jj + u) * stride * 3;
             a[u][0] = p[0];
             a[u][1] = p[1];
             a[u][2] = p[2];
-            c[u] = DENSE ? j + u : Cb[(int64_t)(j + u) * stride];
+            c[u] = DENSE ? jj + u : Cb[(int64_t)(jj + u) * stride];
         }
+    };
+    auto mac = [&](const Group& a, const int (&c)[4]) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             float2 f[NZ];
@@ -75,6 +75,28 @@ __device__ inline void coef_loop(float (&acc)[8][6], const float2* __restrict__ 
             for (int zi = 0; zi < NZ; ++zi) f[zi] = Z[zrow[zi] + c[u]];
 #pragma unroll
             for (int zi = 0; zi < NZ; ++zi) cmac6(acc[zi], a[u][0], a[u][1], a[u][2], f[zi]);
+        }
+    };
+    // Whole groups of 4 terms, software-pipelined over two named buffers:
+    // the next group's loads are in flight while the current one is
+    // accumulated.  The prefetch start is clamped to the last whole group, so
+    // the loads are unconditional (a clamped group is loaded and not used).
+    int j = j0;
+    const int ng = (j1 - j0) >> 2;
+    if (ng > 0) {
+        const int jlast = j0 + 4 * (ng - 1);
+        Group a0, a1;
+        int c0[4], c1[4];
+        load(a0, c0, j);
+        for (int g = 0;;) {
+            load(a1, c1, min(j + 4, jlast));
+            mac(a0, c0);
+            j += 4;
+            if (++g == ng) break;
+            load(a0, c0, min(j + 4, jlast));
+            mac(a1, c1);
+            j += 4;
+            if (++g == ng) break;
         }
     }
     for (; j < j1; ++j) {
@@ -510,131 +532,164 @@ static void launch_main(const scw_exec* ex, dim3 grid, hipStream_t st, int ytps,
 }
 
 // ---------------------------------------------------------------------------
-// MFMA mode contraction for dense eigenmode regions (16-bit operand modes):
-//   C_z[n] = sum_c Z[z][c] A[c][n],   n = real column (cell, comp, re|im)
-// as a real GEMM with the complex product folded into K (k = 2c + s):
-//   A'[2c][n] = A re|im as stored, A'[2c+1][n] = (i A) re|im,
-//   Z'[z][2c] = Re Z, Z'[z][2c+1] = Im Z.
-// MFMA rows are n (4 consecutive per lane), columns z (16 per tile, up to 4
-// tiles), K 32 per step = 16 modes.  Every term is read once per run.
+// MFMA mode contraction for dense eigenmode regions (every cell holds every
+// kz; the term index is the kz index c):
+//   C_z[m] = sum_c Z_z[c] A[c][m],   m = complex column (cell, comp)
+// as a real GEMM D[i][m] = sum_k Z'[i][k] A'[k][m] with rows i = (z, re|im)
+// and k = (c, re|im).  A' is the amplitude table as stored (float2 re, im per
+// (c, m), read once per run in 128-byte row segments); the complex product
+// lives in the small operand Z' (signed parts of Z, precomputed per run in
+// the MFMA A-operand lane layout, split into 16-bit high and low parts for
+// the 16-bit modes).  Rows: 16 per tile = 8 z; the wave owns 64 columns
+// (4 tiles) and MT row tiles, and walks its mode range in steps of 16 modes:
+//   MODE 0..2: v_mfma_f32_16x16x32_{bf16,bf16,f16}, lane k = 8 g + e
+//              <-> (c = 4 g + e / 2, part e % 2), g = lane / 16
+//   MODE 3:    v_mfma_f32_16x16x4_f32, MFMA q = 0..7, lane k = g
+//              <-> (c = g + 4 (q / 2), part q % 2)
+// Either way lane (j, g) reads the float2 A[c][m0 + 16 nt + j] of 4 modes.
 // ---------------------------------------------------------------------------
-template <int MODE, int NZT>
+template <int MODE, int MT>
 __device__ inline void modal_body(const ModalItem& I, const RegionDev& R, const ZMeta* __restrict__ zmeta,
-                                  const float2* __restrict__ Z, int64_t c_elems, float2* __restrict__ C) {
-    typedef typename Op16<MODE>::vec vec;
-    typedef typename Op16<MODE>::elem elem;
-    const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
-    const int64_t ncols = 6LL * R.cells;
-    const float* Ab = reinterpret_cast<const float*>(R.ellA);
-    int64_t zr[NZT];
-    bool zok[NZT];
+                                  const uint32_t* __restrict__ zf, int64_t c_elems, float2* __restrict__ C) {
+    const int lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4;
+    const int mcols = 3 * R.cells;
+    const int W = R.width;
+    const float2* A = R.ellA;
+    int col[4];
 #pragma unroll
-    for (int zt = 0; zt < NZT; ++zt) {
-        const int zz = 16 * zt + li;
-        zok[zt] = zz < I.nz;
-        zr[zt] = zmeta[I.z0 + min(zz, I.nz - 1)].zrow_off;
-    }
-    floatx4 acc[4][NZT];
+    for (int nt = 0; nt < 4; ++nt) col[nt] = min(I.m0 + 16 * nt + j, mcols - 1);
+    floatx4 acc[4][MT];
 #pragma unroll
-    for (int it = 0; it < 4; ++it)
+    for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
-        for (int zt = 0; zt < NZT; ++zt) acc[it][zt] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int mt = 0; mt < MT; ++mt) acc[nt][mt] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 
-    // Raw operands of one 16-mode step; the next step's loads are issued
-    // before the current step's conversions and MFMAs.
-    float2 ra[4][4], rz[NZT][4];
-    auto load_step = [&](int cb) {
+    // Raw operands of one 16-mode step.  Loads are unconditional: modes past
+    // the width re-read the last mode (their Z' entries are zero) and columns
+    // past the table re-read the last column (not stored).
+    typedef float2 RawA[4][4];
+    typedef uint32x4 RawZ[MT][2];
+    const uint32_t* zb = zf + I.zf_off + lane * 8;
+    auto load = [&](RawA& ra, RawZ& rz, int t) {
 #pragma unroll
-        for (int it = 0; it < 4; ++it) {
-            const int64_t n = I.n0 + 16 * it + li;
+        for (int h = 0; h < 4; ++h) {
+            const int c = min(16 * t + (MODE == 3 ? g + 4 * h : 4 * g + h), W - 1);
+            const float2* row = A + (int64_t)c * mcols;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int c = cb + 4 * lk + q;
-                ra[it][q] = (n < ncols && c < I.c1)
-                                ? *reinterpret_cast<const float2*>(Ab + (int64_t)c * ncols + (n & ~(int64_t)1))
-                                : make_float2(0.0f, 0.0f);
-            }
+            for (int nt = 0; nt < 4; ++nt) ra[nt][h] = row[col[nt]];
         }
 #pragma unroll
-        for (int zt = 0; zt < NZT; ++zt)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int c = cb + 4 * lk + q;
-                rz[zt][q] = (zok[zt] && c < I.c1) ? Z[zr[zt] + c] : make_float2(0.0f, 0.0f);
-            }
+        for (int mt = 0; mt < MT; ++mt) {
+            const uint32_t* p = zb + ((int64_t)mt * I.T + t) * 512;
+            rz[mt][0] = *reinterpret_cast<const uint32x4*>(p);
+            if constexpr (MODE == 1 || MODE == 3) rz[mt][1] = *reinterpret_cast<const uint32x4*>(p + 4);
+        }
     };
-    if (I.c0 < I.c1) load_step(I.c0);
-    for (int cb = I.c0; cb < I.c1; cb += 16) {
-        vec ah[4], al[4], zh[NZT], zl[NZT];
+    auto step = [&](const RawA& ra, const RawZ& rz) {
+        if constexpr (MODE == 3) {
 #pragma unroll
-        for (int it = 0; it < 4; ++it) {
-            const bool odd = ((I.n0 + 16 * it + li) & 1) != 0;
+            for (int q = 0; q < 8; ++q)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const float2 u = ra[it][q];
-                const float v0 = odd ? u.y : u.x;   // k = 2c:   A re (n even) / A im (n odd)
-                const float v1 = odd ? u.x : -u.y;  // k = 2c+1: -A im (n even) / A re (n odd)
-                const elem h0 = (elem)v0, h1 = (elem)v1;
-                ah[it][2 * q] = h0;
-                ah[it][2 * q + 1] = h1;
-                if constexpr (MODE == 1) {
-                    al[it][2 * q] = (elem)(v0 - (float)h0);
-                    al[it][2 * q + 1] = (elem)(v1 - (float)h1);
+                for (int nt = 0; nt < 4; ++nt) {
+                    const float2 u = ra[nt][q >> 1];
+                    const float bv = (q & 1) ? u.y : u.x;
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt) {
+                        const uint32_t w = rz[mt][q >> 2][q & 3];  // element copy first: bit-casting a
+                        const float av = __builtin_bit_cast(float, w);  // vector element reads element 0
+                        acc[nt][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[nt][mt], 0, 0, 0);
+                    }
+                }
+        } else {
+            typedef typename Op16<MODE>::vec vec;
+            typedef typename Op16<MODE>::elem elem;
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                vec bh, bl;
+#pragma unroll
+                for (int h = 0; h < 4; ++h) {
+                    const float2 u = ra[nt][h];
+                    const elem h0 = (elem)u.x, h1 = (elem)u.y;
+                    bh[2 * h] = h0;
+                    bh[2 * h + 1] = h1;
+                    if constexpr (MODE == 1) {
+                        bl[2 * h] = (elem)(u.x - (float)h0);
+                        bl[2 * h + 1] = (elem)(u.y - (float)h1);
+                    }
+                }
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt) {
+                    const vec zh = __builtin_bit_cast(vec, rz[mt][0]);
+                    if constexpr (MODE == 1) {
+                        const vec zl = __builtin_bit_cast(vec, rz[mt][1]);
+                        acc[nt][mt] = mfma16<MODE>(zh, bl, acc[nt][mt]);
+                        acc[nt][mt] = mfma16<MODE>(zl, bh, acc[nt][mt]);
+                    }
+                    acc[nt][mt] = mfma16<MODE>(zh, bh, acc[nt][mt]);
                 }
             }
         }
-#pragma unroll
-        for (int zt = 0; zt < NZT; ++zt) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const float2 f = rz[zt][q];
-                const elem h0 = (elem)f.x, h1 = (elem)f.y;
-                zh[zt][2 * q] = h0;
-                zh[zt][2 * q + 1] = h1;
-                if constexpr (MODE == 1) {
-                    zl[zt][2 * q] = (elem)(f.x - (float)h0);
-                    zl[zt][2 * q + 1] = (elem)(f.y - (float)h1);
-                }
-            }
-        }
-        if (cb + 16 < I.c1) load_step(cb + 16);
-#pragma unroll
-        for (int it = 0; it < 4; ++it)
-#pragma unroll
-            for (int zt = 0; zt < NZT; ++zt) {
-                if constexpr (MODE == 1) {
-                    acc[it][zt] = mfma16<MODE>(al[it], zh[zt], acc[it][zt]);
-                    acc[it][zt] = mfma16<MODE>(ah[it], zl[zt], acc[it][zt]);
-                }
-                acc[it][zt] = mfma16<MODE>(ah[it], zh[zt], acc[it][zt]);
-            }
+    };
+    // Two named buffers, the next step's loads in flight while the current
+    // step computes; prefetch indices are clamped to the last step.
+    const int t0 = I.c0 >> 4, t1 = I.c1 >> 4;
+    RawA a0, a1;
+    RawZ z0, z1;
+    load(a0, z0, t0);
+    for (int t = t0;;) {
+        load(a1, z1, min(t + 1, t1 - 1));
+        step(a0, z0);
+        if (++t == t1) break;
+        load(a0, z0, min(t + 1, t1 - 1));
+        step(a1, z1);
+        if (++t == t1) break;
     }
-    // lane holds D[n = n0 + 16 it + 4 lk + r][z = 16 zt + li]
+    // lane holds D[i = 4 g + r][m0 + 16 nt + j]: rows 4g..4g+3 are (z, re),
+    // (z, im), (z + 1, re), (z + 1, im) with z = 8 mt + 2 g.
 #pragma unroll
-    for (int zt = 0; zt < NZT; ++zt) {
-        if (!zok[zt]) continue;
-        float* o = reinterpret_cast<float*>(C + (int64_t)I.slot * c_elems + zmeta[I.z0 + 16 * zt + li].c_off);
+    for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-        for (int it = 0; it < 4; ++it) {
-            const int64_t n = I.n0 + 16 * it + 4 * lk;
-            if (n + 1 < ncols) *reinterpret_cast<float2*>(o + n) = make_float2(acc[it][zt][0], acc[it][zt][1]);
-            if (n + 3 < ncols) *reinterpret_cast<float2*>(o + n + 2) = make_float2(acc[it][zt][2], acc[it][zt][3]);
+        for (int h = 0; h < 2; ++h) {
+            const int zi = 8 * mt + 2 * g + h;
+            if (zi >= I.nz) continue;
+            float2* o = C + (int64_t)I.slot * c_elems + zmeta[I.z0 + zi].c_off;
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                const int m = I.m0 + 16 * nt + j;
+                if (m < mcols) o[m] = make_float2(acc[nt][mt][2 * h], acc[nt][mt][2 * h + 1]);
+            }
         }
-    }
 }
 
-template <int MODE>
+template <int MODE, int MT>
 __global__ __launch_bounds__(256) void modal_kernel(const ModalItem* __restrict__ items, int nitems,
                                                     const RegionDev* __restrict__ regions,
-                                                    const ZMeta* __restrict__ zmeta, const float2* __restrict__ Z,
+                                                    const ZMeta* __restrict__ zmeta, const uint32_t* __restrict__ zf,
                                                     int64_t c_elems, float2* __restrict__ C) {
     const int it = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
     if (it >= nitems) return;
     const ModalItem I = items[it];
-    const RegionDev R = regions[I.region];
-    if (I.nz <= 16) modal_body<MODE, 1>(I, R, zmeta, Z, c_elems, C);
-    else if (I.nz <= 32) modal_body<MODE, 2>(I, R, zmeta, Z, c_elems, C);
-    else modal_body<MODE, 4>(I, R, zmeta, Z, c_elems, C);
+    modal_body<MODE, MT>(I, regions[I.region], zmeta, zf, c_elems, C);
+}
+
+template <int MODE>
+static void launch_modal(const scw_exec* ex, hipStream_t st) {
+    const ModalItem* mi = ex->d_mitems.as<ModalItem>();
+    const RegionDev* rg = ex->d_regions.as<RegionDev>();
+    const ZMeta* zm = ex->d_zmeta.as<ZMeta>();
+    const uint32_t* zf = ex->d_Zf.as<uint32_t>();
+    float2* C = ex->d_C.as<float2>();
+    for (int k = 0; k < 4; ++k) {
+        const int64_t off = ex->mclass_off[k], n = ex->mclass_off[k + 1] - off;
+        if (n <= 0) continue;
+        const dim3 g((unsigned)((n + 3) / 4));
+        switch (k) {
+            case 0: hipLaunchKernelGGL((modal_kernel<MODE, 1>), g, dim3(256), 0, st, mi + off, (int)n, rg, zm, zf, ex->c_elems, C); break;
+            case 1: hipLaunchKernelGGL((modal_kernel<MODE, 2>), g, dim3(256), 0, st, mi + off, (int)n, rg, zm, zf, ex->c_elems, C); break;
+            case 2: hipLaunchKernelGGL((modal_kernel<MODE, 4>), g, dim3(256), 0, st, mi + off, (int)n, rg, zm, zf, ex->c_elems, C); break;
+            default: hipLaunchKernelGGL((modal_kernel<MODE, 8>), g, dim3(256), 0, st, mi + off, (int)n, rg, zm, zf, ex->c_elems, C); break;
+        }
+    }
 }
 
 // Stages: 1 = coefficient contraction (VALU ELL items and MFMA mode items),
@@ -650,19 +705,10 @@ int launch_gemm(const scw_exec* ex, float* out, hipStream_t st, int stage) {
             SCW_HIP(hipGetLastError());
         }
         if (nmitems > 0) {
-            const dim3 g((unsigned)((nmitems + 3) / 4));
-            const ModalItem* mi = ex->d_mitems.as<ModalItem>();
-            const RegionDev* rg = ex->d_regions.as<RegionDev>();
-            const ZMeta* zm = ex->d_zmeta.as<ZMeta>();
-            if (ex->dtype == SCW_DTYPE_BF16)
-                hipLaunchKernelGGL(modal_kernel<0>, g, dim3(256), 0, st, mi, nmitems, rg, zm, ex->d_Z.as<float2>(),
-                                   ex->c_elems, ex->d_C.as<float2>());
-            else if (ex->dtype == SCW_DTYPE_BF16X3)
-                hipLaunchKernelGGL(modal_kernel<1>, g, dim3(256), 0, st, mi, nmitems, rg, zm, ex->d_Z.as<float2>(),
-                                   ex->c_elems, ex->d_C.as<float2>());
-            else
-                hipLaunchKernelGGL(modal_kernel<2>, g, dim3(256), 0, st, mi, nmitems, rg, zm, ex->d_Z.as<float2>(),
-                                   ex->c_elems, ex->d_C.as<float2>());
+            if (ex->dtype == SCW_DTYPE_FP32) launch_modal<3>(ex, st);
+            else if (ex->dtype == SCW_DTYPE_BF16) launch_modal<0>(ex, st);
+            else if (ex->dtype == SCW_DTYPE_BF16X3) launch_modal<1>(ex, st);
+            else launch_modal<2>(ex, st);
             SCW_HIP(hipGetLastError());
         }
     }

@@ -112,13 +112,14 @@ struct CoefItem {
 // One wave-sized work item of the MFMA mode contraction (dense regions):
 // C_z[n] += sum_{c in [c0, c1)} Z[z][c] A[c][n] for real columns n in
 // [n0, n0 + 64) and the run z0 .. z0 + nz - 1 (nz <= 64).
-struct ModalItem {
-    int32_t region;
-    int32_t z0, nz;
-    int32_t n0;
-    int32_t c0, c1;
-    int32_t slot;
-    int32_t pad;
+struct ModalItem {       // one wave of the MFMA mode contraction
+    int32_t region;      // virtual region
+    int32_t z0, nz;      // z run (nz <= 32)
+    int32_t m0;          // first complex column (cell * 3 + comp); the wave covers 64
+    int32_t c0, c1;      // mode range, multiples of 16 (c1 may pass the width)
+    int32_t slot;        // partial-sum slot
+    int32_t T;           // 16-mode steps in the run's phase-fragment table
+    int64_t zf_off;      // 32-bit word offset of the run's phase fragments
 };
 
 struct RegionDev {      // per-region data the GEMM-form kernels read (device)
@@ -156,8 +157,9 @@ struct scw_exec {
     int my_max = 0;
     int nslots_max = 1;
     std::vector<scw::CoefItem> items;
-    std::vector<scw::ModalItem> mitems;
-    scw::DevBuf d_items, d_mitems, d_regions, d_zmeta, d_Z, d_X, d_C, d_U, d_Yfrag;
+    std::vector<scw::ModalItem> mitems;   // grouped by row-tile class (MT = 1, 2, 4, 8)
+    int64_t mclass_off[5] = {0, 0, 0, 0, 0};
+    scw::DevBuf d_items, d_mitems, d_regions, d_zmeta, d_Z, d_Zf, d_X, d_C, d_U, d_Yfrag;
     int64_t tune_coef_items = 0;   // launch-shape choices (0 = default)
     int64_t tune_main_blocks = 0;
     std::vector<double> hx, hy;    // grid kept for re-planning launch shapes

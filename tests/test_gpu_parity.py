@@ -128,6 +128,31 @@ def test_gemm_ragged_grids(tile_small, dtype, nx, ny):
 
 
 @pytest.fixture(scope="module")
+def slab_dense():
+    cw = configs.tile_sample_structure(M=7, Z=1, size=(12.0, 12.0, 2.0), lam=0.7, seed=3, incidence_deg=12.0)
+    t = cw.unpack()
+    # 75 z inside the slab: one stretch of the dense region, split into runs
+    # of 25 (4 row tiles of the MFMA mode contraction); 5 z outside it
+    zs = np.concatenate([configs.axis(-0.95, 0.95, 75), configs.axis(1.05, 1.6, 5)])
+    xs, ys = configs.axis(-6, 6, 18), configs.axis(-6, 6, 14)
+    ref = oracle.eval_grid_f64(t.W, t.z_layers, t.begin, t.end, xs, ys, zs)
+    return Plan(t), (xs, ys, zs), ref
+
+
+@pytest.mark.parametrize("dtype", ["fp32", "bf16x3", "bf16", "fp16"])
+def test_dense_slab_mode_contraction(slab_dense, dtype, monkeypatch):
+    """Long z stretches through a dense eigenmode region: the MFMA mode
+    contraction (runs of up to 32 z) and the VALU term contraction (forced
+    with SCW_MODAL_MIN_RUN) both match the oracle."""
+    plan, g, ref = slab_dense
+    info = plan.region_info(1)  # every cell holds every kz: dense
+    assert info["width"] == info["mc"] and info["n_terms"] == info["mx"] * info["my"] * info["mc"]
+    _check(Exec(plan, *g, form="gemm", dtype=dtype).run_host(), ref, TOL[dtype], f"dense slab modal {dtype}")
+    monkeypatch.setenv("SCW_MODAL_MIN_RUN", "1000000")
+    _check(Exec(plan, *g, form="gemm", dtype=dtype).run_host(), ref, TOL[dtype], f"dense slab VALU {dtype}")
+
+
+@pytest.fixture(scope="module")
 def stack_small():
     rng = np.random.default_rng(11)
     n = np.concatenate([[1.0], rng.uniform(1, 2, 5) + 1j * rng.uniform(0, 0.05, 5)])
