@@ -20,6 +20,7 @@ rel-L2 figures are measured against.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--direct-steps D]
 """
 import argparse
+import re
 import collections
 import json
 import os
@@ -79,20 +80,35 @@ def time_stages(ex, out, stream, reps):
     return [t / reps for t in tot]
 
 
-def profiled_traffic(kernel_prefix):
-    """HBM bytes per dispatch of a kernel from the committed rocprofv3 summary
-    (FETCH_SIZE / WRITE_SIZE passes scaled against a plain copy), or None."""
+def profiled_traffic(patterns):
+    """HBM bytes per step of the kernels matching `patterns` (regular
+    expressions on the kernel name), from the committed rocprofv3 summary of
+    the tile-sample workload (FETCH_SIZE / WRITE_SIZE per dispatch, scaled
+    against a plain copy); None if the summary lists none of them."""
     path = os.path.join(ROOT, "profiles", "tile_sample_c2_kernels.md")
     if not os.path.exists(path):
         return None
+    total, found = 0.0, False
     for line in open(path):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 6 and cells[0].strip("`").startswith(kernel_prefix):
+        if len(cells) != 5:
+            continue
+        name = cells[0].strip("`")
+        if any(re.fullmatch(p, name) for p in patterns):
             try:
-                return float(cells[5]) * 1e6
+                total += float(cells[4]) * 1e6
+                found = True
             except ValueError:
                 return None
-    return None
+    return total if found else None
+
+
+def stage_kernels(stage, dt):
+    """Kernel-name patterns of one GEMM-form stage in one dtype."""
+    mode = {"fp32": 3, "bf16": 0, "bf16x3": 1, "fp16": 2}[dt]
+    main = r"scw::main_f32<\d+, (true|false)>" if dt == "fp32" else r"scw::main_16<\d+, (true|false), %d>" % mode
+    return {"coef": [r"scw::coef_kernel", r"scw::modal_kernel<%d, \d>" % mode],
+            "xcontract": [r"scw::u_kernel"], "main": [main]}[stage]
 
 
 def roofline(flops, nbytes, ms, peak_flops):
@@ -259,11 +275,10 @@ def main():
     S = B["stages"][dom]
     peak = PEAK["fp32"] if dt == "fp32" or dom != "main" else PEAK["bf16"]
     roof = roofline(S["flops"], S["bytes"], S["ms"], peak)
-    kname = {"coef": "scw::coef_kernel", "xcontract": "scw::u_kernel",
-             "main": "scw::main_f32" if dt == "fp32" else
-             "scw::main_16<64, true, %d>" % {"bf16": 0, "bf16x3": 1, "fp16": 2}[dt]}[dom]
-    roof["kernel"] = kname
-    roof["traffic"] = profiled_traffic(kname) if args.workload == "c2" else None
+    kpat = stage_kernels(dom, dt)
+    roof["kernel"] = " + ".join(kpat)
+    roof["stage"] = dom
+    roof["traffic"] = profiled_traffic(kpat) if args.workload == "c2" else None
     D = forms["direct_fp32"]
     roof_direct = {"bound": "valu", "achieved": round(D["info"]["flops"] / (D["ms"] * 1e-3) / 1e12, 2),
                    "peak": PEAK["fp32"], "unit": "TFLOP/s",
@@ -368,7 +383,8 @@ def bench_batch(args):
     dom = max(B["stages"], key=lambda k: B["stages"][k]["ms"])
     S = B["stages"][dom]
     roof = roofline(S["flops"], S["bytes"], S["ms"], PEAK["fp32"] if dt == "fp32" else PEAK["bf16"])
-    roof["kernel"] = {"coef": "scw::modal_kernel", "xcontract": "scw::u_kernel", "main": "scw::main"}[dom]
+    roof["kernel"] = " + ".join(stage_kernels(dom, dt))
+    roof["stage"] = dom
     line = {
         "metric": METRIC, "value": B["rate"], "unit": UNIT, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": B["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dt,

@@ -148,7 +148,8 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
         if (!R.separable)
             return fail(SCW_ERR_UNSUPPORTED, "GEMM form: region %d is not separable (%s)", r, R.why_not.c_str());
         RegionDev& d = rd[v];
-        d.ellA = P->d_ellA.as<float2>() + R.ell_off * 3;
+        d.ellA = P->d_ellA.as<float2>() + R.ell_off;
+        d.arow = (int32_t)R.arow;
         d.ellC = R.ellc_off >= 0 ? P->d_ellC.as<int32_t>() + R.ellc_off : nullptr;
         d.cells = R.mx * R.my;
         d.width = R.width;

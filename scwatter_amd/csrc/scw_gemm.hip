@@ -54,13 +54,13 @@ __device__ inline void cmac6(float* acc, const float2 a0, const float2 a1, const
 // scalar cache.
 template <int NZ, bool DENSE>
 __device__ inline void coef_loop(float (&acc)[8][6], const float2* __restrict__ Ab,
-                                 const int32_t* __restrict__ Cb, int64_t stride, const float2* __restrict__ Z,
-                                 const int64_t (&zrow)[8], int j0, int j1) {
+                                 const int32_t* __restrict__ Cb, int64_t stride, int64_t arow,
+                                 const float2* __restrict__ Z, const int64_t (&zrow)[8], int j0, int j1) {
     typedef float2 Group[4][3];
     auto load = [&](Group& a, int (&c)[4], int jj) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const float2* p = Ab + (int64_t)(jj + u) * stride * 3;
+            const float2* p = Ab + (int64_t)(jj + u) * arow;
             a[u][0] = p[0];
             a[u][1] = p[1];
             a[u][2] = p[2];
@@ -100,7 +100,7 @@ __device__ inline void coef_loop(float (&acc)[8][6], const float2* __restrict__ 
         }
     }
     for (; j < j1; ++j) {
-        const float2* p = Ab + (int64_t)j * stride * 3;
+        const float2* p = Ab + (int64_t)j * arow;
         const float2 a0 = p[0], a1 = p[1], a2 = p[2];
         const int c = DENSE ? j : Cb[(int64_t)j * stride];
 #pragma unroll
@@ -132,11 +132,11 @@ __global__ __launch_bounds__(256) void coef_kernel(const CoefItem* __restrict__ 
 
     const float2* Ab = R.ellA + (int64_t)cl * 3;
     const int32_t* Cb = R.dense ? nullptr : R.ellC + cl;
-    const int64_t stride = (int64_t)R.cells;
+    const int64_t stride = (int64_t)R.cells, arow = R.arow;
 #define SCW_COEF_CASE(N)                                                                   \
     case N:                                                                                \
-        if (R.dense) coef_loop<N, true>(acc, Ab, Cb, stride, Z, zrow, I.j0, I.j1);        \
-        else coef_loop<N, false>(acc, Ab, Cb, stride, Z, zrow, I.j0, I.j1);               \
+        if (R.dense) coef_loop<N, true>(acc, Ab, Cb, stride, arow, Z, zrow, I.j0, I.j1);        \
+        else coef_loop<N, false>(acc, Ab, Cb, stride, arow, Z, zrow, I.j0, I.j1);               \
         break;
     switch (nz) {
         SCW_COEF_CASE(1)
@@ -147,8 +147,8 @@ __global__ __launch_bounds__(256) void coef_kernel(const CoefItem* __restrict__ 
         SCW_COEF_CASE(6)
         SCW_COEF_CASE(7)
         default:
-            if (R.dense) coef_loop<8, true>(acc, Ab, Cb, stride, Z, zrow, I.j0, I.j1);
-            else coef_loop<8, false>(acc, Ab, Cb, stride, Z, zrow, I.j0, I.j1);
+            if (R.dense) coef_loop<8, true>(acc, Ab, Cb, stride, arow, Z, zrow, I.j0, I.j1);
+            else coef_loop<8, false>(acc, Ab, Cb, stride, arow, Z, zrow, I.j0, I.j1);
             break;
     }
 #undef SCW_COEF_CASE
@@ -574,7 +574,7 @@ __device__ inline void modal_body(const ModalItem& I, const RegionDev& R, const 
 #pragma unroll
         for (int h = 0; h < 4; ++h) {
             const int c = min(16 * t + (MODE == 3 ? g + 4 * h : 4 * g + h), W - 1);
-            const float2* row = A + (int64_t)c * mcols;
+            const float2* row = A + (int64_t)c * R.arow;
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt) ra[nt][h] = row[col[nt]];
         }

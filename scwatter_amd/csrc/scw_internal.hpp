@@ -61,7 +61,8 @@ struct RegionPlan {
     int mx = 0, my = 0, mc = 0, width = 0;
     int64_t n_terms = 0;
     std::vector<cd> kx, ky, kz;    // distinct values
-    int64_t ell_off = 0;           // slot offset of this region's ELL amplitudes (slot = j*cells + cell)
+    int64_t ell_off = 0;           // float2 offset of this region's ELL amplitudes
+    int64_t arow = 0;              // float2 per term row: 3 cells rounded up to 16 (128-byte rows)
     int64_t ellc_off = -1;         // slot offset of its kz indices (-1: dense, index = j)
     std::string why_not;           // reason when not separable
 };
@@ -80,7 +81,7 @@ struct scw_plan {
     scw::DevBuf d_lns;   // double[n]     ln s  (s = max_c |E0_c|, 0 -> wave skipped)
     scw::DevBuf d_kf;    // float4[n]     Re kx/2pi, Re ky/2pi, -Im kx log2e, -Im ky log2e
     // GEMM-form tables, concatenated over separable regions
-    scw::DevBuf d_ellA;  // float2[...][width][cells][3]  E0 re-referenced to zref
+    scw::DevBuf d_ellA;  // float2[...][width][arow]: [cells][3] E0 re-referenced to zref, padded rows
     scw::DevBuf d_ellC;  // int32[...][width][cells]      kz index (sparse regions only)
     int64_t ell_elems = 0;
 };
@@ -123,11 +124,11 @@ struct ModalItem {       // one wave of the MFMA mode contraction
 };
 
 struct RegionDev {      // per-region data the GEMM-form kernels read (device)
-    const float2* ellA; // the region's terms: [width][cells][3]
+    const float2* ellA; // the region's terms: [width][arow], row j = [cells][3] float2
     const int32_t* ellC;// kz index per term slot (nullptr when dense)
     int64_t x_off;      // float2 offset of X_r ([mx][nx])
     int32_t cells, width, mc, dense;
-    int32_t mx, my, nslots, pad;
+    int32_t mx, my, nslots, arow;
 };
 
 // Direct form helpers (scw_direct.hip).

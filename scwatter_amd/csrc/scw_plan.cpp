@@ -131,7 +131,7 @@ static const int MX_MAX = 128;  // largest kx count (LDS staging of the X contra
 
 static void plan_region(RegionPlan& R, const scw_planewave64* W, const std::vector<double>& lns,
                         const std::vector<cd>& ehat, std::vector<float>& ellA, std::vector<int32_t>& ellC,
-                        int64_t& ellA_slots, int64_t& ellC_slots) {
+                        int64_t& ellA_slots, int64_t& ellC_slots) {  // ellA_slots: float2 count
     const int64_t n = R.wend - R.wbegin;
     R.separable = false;
     R.dense = false;
@@ -141,6 +141,7 @@ static void plan_region(RegionPlan& R, const scw_planewave64* W, const std::vect
         R.mx = R.my = R.mc = R.width = 0;
         R.n_terms = 0;
         R.ell_off = ellA_slots;
+        R.arow = 0;
         return;
     }
     std::vector<cd> kx(n), ky(n), kz(n);
@@ -242,9 +243,12 @@ static void plan_region(RegionPlan& R, const scw_planewave64* W, const std::vect
                 return;
             }
     R.separable = true;
+    // Term rows padded to whole 128-byte lines, so every row starts on a line
+    // and the contractions' 128-byte row segments never straddle two.
+    R.arow = (3 * cells + 15) / 16 * 16;
     R.ell_off = ellA_slots;
     const size_t baseA = ellA.size();
-    ellA.resize(baseA + (size_t)slots * 6, 0.0f);
+    ellA.resize(baseA + (size_t)width * R.arow * 2, 0.0f);
     if (!dense) {
         R.ellc_off = ellC_slots;
         ellC.resize(ellC.size() + (size_t)slots, 0);
@@ -253,14 +257,14 @@ static void plan_region(RegionPlan& R, const scw_planewave64* W, const std::vect
     for (auto& t : terms) {
         const int j = dense ? t.c : fill[t.cell]++;
         const int64_t slot = (int64_t)j * cells + t.cell;
-        float* a = &ellA[baseA + (size_t)slot * 6];
+        float* a = &ellA[baseA + ((size_t)j * R.arow + (size_t)t.cell * 3) * 2];
         for (int c = 0; c < 3; ++c) {
             a[2 * c] = (float)t.e[c].real();
             a[2 * c + 1] = (float)t.e[c].imag();
         }
         if (!dense) ellC[(size_t)(R.ellc_off + slot)] = t.c;
     }
-    ellA_slots += slots;
+    ellA_slots += (int64_t)width * R.arow;
     if (!dense) ellC_slots += slots;
 }
 
