@@ -190,11 +190,21 @@ __global__ __launch_bounds__(256) void u_kernel(const ZMeta* __restrict__ zmeta,
         const int g = G > 1 ? (int)threadIdx.x / m : 0;
         float2 s = make_float2(0.0f, 0.0f);
         if (t < m && g < G) {
+            // slots g, g + G, ... added in that order; loads issued 4 at a time
             const int64_t off = zm.c_off + ((int64_t)b * R.mx + t / 3) * 3 + (t % 3);
-            for (int k = g; k < R.nslots; k += G) {
-                const float2 v = C[(int64_t)k * c_elems + off];
-                s.x += v.x;
-                s.y += v.y;
+            for (int k0 = g; k0 < R.nslots; k0 += 4 * G) {
+                float2 v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int k = k0 + u * G;
+                    v[u] = k < R.nslots ? C[(int64_t)k * c_elems + off] : make_float2(0.0f, 0.0f);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (k0 + u * G < R.nslots) {
+                        s.x += v[u].x;
+                        s.y += v[u].y;
+                    }
             }
         }
         if (G > 1) {
@@ -218,8 +228,18 @@ __global__ __launch_bounds__(256) void u_kernel(const ZMeta* __restrict__ zmeta,
     if (x >= nx) return;
     float acc[6] = {0, 0, 0, 0, 0, 0};
     const float2* Xr = X + R.x_off + x;
-    for (int a = 0; a < R.mx; ++a)
-        cmac6(acc, sc[3 * a], sc[3 * a + 1], sc[3 * a + 2], Xr[(int64_t)a * nx]);
+    // X values 8 at a time: the loads of a chunk are in flight together
+    for (int a0 = 0; a0 < R.mx; a0 += 8) {
+        float2 xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) xv[u] = a0 + u < R.mx ? Xr[(int64_t)(a0 + u) * nx] : make_float2(0.0f, 0.0f);
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (a0 + u < R.mx) {
+                const int a = a0 + u;
+                cmac6(acc, sc[3 * a], sc[3 * a + 1], sc[3 * a + 2], xv[u]);
+            }
+    }
     float2* o = U + zm.u_off + ((int64_t)b * nx + x) * 3;
     o[0] = make_float2(acc[0], acc[1]);
     o[1] = make_float2(acc[2], acc[3]);
