@@ -12,7 +12,7 @@
 // Three launches per grid:
 //   coef_kernel : C_z for every z of the grid (ELL contraction over the
 //                 region's terms; wave-level work items, FP32 VALU).
-//   u_kernel    : U_z[b][x][comp] = sum_a C_z[a,b,comp] X[a][x]  (FP32 VALU).
+//   u_mfma      : U_z[b][x][comp] = sum_a C_z[a,b,comp] X[a][x]  (MFMA).
 //   main_*      : E_z[y][x][comp] = sum_b Y[y,b] U_z[b][x][comp] on MFMA.
 //
 // The main GEMM is a real GEMM with the complex product folded into K:
@@ -164,86 +164,139 @@ __global__ __launch_bounds__(256) void coef_kernel(const CoefItem* __restrict__ 
     }
 }
 
-// ---------------------------------------------------------------------------
-// U_z[b][x][comp] = sum_a (sum_slots C_z[a,b,comp]) X[a][x]
-// The block (z, b) first adds the partial-sum slots of C_z[:, b, :] into LDS
-// (fixed slot order), then each thread contracts one x against them.
-// ---------------------------------------------------------------------------
 constexpr int MX_MAX = 128;
 
-__global__ __launch_bounds__(256) void u_kernel(const ZMeta* __restrict__ zmeta, const RegionDev* __restrict__ regions,
-                                                const float2* __restrict__ C, int64_t c_elems,
-                                                const float2* __restrict__ X, int nx, float2* __restrict__ U, int zoff) {
-    __shared__ float2 sc[MX_MAX * 3];
-    __shared__ float2 part[256];
-    const int z = blockIdx.z + zoff, b = blockIdx.y;
+// ---------------------------------------------------------------------------
+// X contraction on MFMA: per z, U_z[b][x][comp] = sum_a C_z[a,b,comp] X[a][x]
+// is a real GEMM with rows i = (m = 3 b + comp, re|im), K = (a, re|im) and
+// columns x.  The A operand holds the signed parts of C (re,re = Re C,
+// re,im = -Im C, im,re = Im C, im,im = Re C), taken from LDS after the
+// partial-sum slots are added in a fixed order; the B operand is X as stored.
+// K mapping per 16 a as in the mode contraction (MODE 1: lane k = 8 g + e <->
+// a = 4 g + e / 2; MODE 3: MFMA q, lane k = g <-> a = g + 4 (q / 2)).  MODE 1
+// (bf16 x3) serves the 16-bit dtypes, MODE 3 (FP32) the FP32 dtype.
+// Workgroup = 8 b (24 rows m) x 256 x of one z; wave = 3 row tiles x 64 x.
+// ---------------------------------------------------------------------------
+constexpr int UB = 8;              // b per workgroup
+constexpr int UPITCH = MX_MAX + 2; // LDS row pitch in float2 (rows stay 16-byte aligned)
+
+template <int MODE>
+__global__ __launch_bounds__(256) void u_mfma(const ZMeta* __restrict__ zmeta, const RegionDev* __restrict__ regions,
+                                              const float2* __restrict__ C, int64_t c_elems,
+                                              const float2* __restrict__ X, int nx, float2* __restrict__ U, int zoff) {
+    __shared__ float2 cs[3 * UB][UPITCH];
+    const int z = blockIdx.z + zoff;
     const ZMeta zm = zmeta[z];
-    if (b >= zm.my) return;
+    const int b0 = blockIdx.y * UB;
+    if (b0 >= zm.my) return;
     const RegionDev R = regions[zm.region];
-    // Sum the partial-sum slots of C_z[:, b, :]: entry t = (a, comp) of m = 3 mx
-    // entries; when m < 128 the threads form G = 256 / m groups that each add
-    // every G-th slot, and the groups are then added in group order.
-    const int m = R.mx * 3;
-    const int G = m < 128 ? 256 / m : 1;
-    for (int t0 = 0; t0 < m; t0 += 256) {
-        const int t = t0 + (int)threadIdx.x % (G > 1 ? m : 256);
-        const int g = G > 1 ? (int)threadIdx.x / m : 0;
-        float2 s = make_float2(0.0f, 0.0f);
-        if (t < m && g < G) {
-            // slots g, g + G, ... added in that order; loads issued 4 at a time
-            const int64_t off = zm.c_off + ((int64_t)b * R.mx + t / 3) * 3 + (t % 3);
-            for (int k0 = g; k0 < R.nslots; k0 += 4 * G) {
-                float2 v[4];
+    const int mx = R.mx, T = (mx + 15) >> 4, ka = 16 * T;
+    // C_z[a][b0 + r / 3][r % 3], slots added in order (16 loads in flight);
+    // zero past mx and past my.
+    for (int e = threadIdx.x; e < 3 * UB * ka; e += 256) {
+        const int r = e / ka, a = e - r * ka;
+        const int b = b0 + r / 3;
+        float2 sum = make_float2(0.0f, 0.0f);
+        if (a < mx && b < zm.my) {
+            const int64_t off = zm.c_off + ((int64_t)b * mx + a) * 3 + (r % 3);
+            for (int k0 = 0; k0 < R.nslots; k0 += 16) {
+                float2 v[16];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int k = k0 + u * G;
-                    v[u] = k < R.nslots ? C[(int64_t)k * c_elems + off] : make_float2(0.0f, 0.0f);
-                }
+                for (int u = 0; u < 16; ++u)
+                    v[u] = k0 + u < R.nslots ? C[(int64_t)(k0 + u) * c_elems + off] : make_float2(0.0f, 0.0f);
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    if (k0 + u * G < R.nslots) {
-                        s.x += v[u].x;
-                        s.y += v[u].y;
+                for (int u = 0; u < 16; ++u)
+                    if (k0 + u < R.nslots) {
+                        sum.x += v[u].x;
+                        sum.y += v[u].y;
                     }
             }
         }
-        if (G > 1) {
-            if (g < G) part[threadIdx.x] = s;
-            __syncthreads();
-            if ((int)threadIdx.x < m) {
-                float2 tot = part[threadIdx.x];
-                for (int q = 1; q < G; ++q) {
-                    const float2 v = part[q * m + threadIdx.x];
-                    tot.x += v.x;
-                    tot.y += v.y;
-                }
-                sc[threadIdx.x] = tot;
-            }
-        } else if (t < m) {
-            sc[t] = s;
-        }
+        cs[r][a] = sum;
     }
     __syncthreads();
-    const int x = blockIdx.x * 256 + threadIdx.x;
-    if (x >= nx) return;
-    float acc[6] = {0, 0, 0, 0, 0, 0};
-    const float2* Xr = X + R.x_off + x;
-    // X values 8 at a time: the loads of a chunk are in flight together
-    for (int a0 = 0; a0 < R.mx; a0 += 8) {
-        float2 xv[8];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4;
+    const int x0 = blockIdx.x * 256 + wv * 64;
+    if (x0 >= nx) return;
+    int xc[4];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) xv[u] = a0 + u < R.mx ? Xr[(int64_t)(a0 + u) * nx] : make_float2(0.0f, 0.0f);
+    for (int nt = 0; nt < 4; ++nt) xc[nt] = min(x0 + 16 * nt + j, nx - 1);
+    const float2* Xr = X + R.x_off;
+    floatx4 acc[4][3];
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
-            if (a0 + u < R.mx) {
-                const int a = a0 + u;
-                cmac6(acc, sc[3 * a], sc[3 * a + 1], sc[3 * a + 2], xv[u]);
+    for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+        for (int mt = 0; mt < 3; ++mt) acc[nt][mt] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    const int o = j & 1;  // row i = lane % 16: m = 8 mt + i / 2, part o
+    for (int t = 0; t < T; ++t) {
+        float2 xa[4][4], cv[3][4];
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            const int a = 16 * t + (MODE == 3 ? g + 4 * h : 4 * g + h);
+            const float2* xrow = Xr + (int64_t)min(a, mx - 1) * nx;
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) xa[nt][h] = xrow[xc[nt]];
+#pragma unroll
+            for (int mt = 0; mt < 3; ++mt) cv[mt][h] = cs[8 * mt + (j >> 1)][a];
+        }
+        // signed part of C for (h, s): o = 0 -> (Re, -Im), o = 1 -> (Im, Re)
+        auto cpart = [&](const float2 c, int s) { return o == 0 ? (s == 0 ? c.x : -c.y) : (s == 0 ? c.y : c.x); };
+        if constexpr (MODE == 3) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt) {
+                    const float bv = (q & 1) ? xa[nt][q >> 1].y : xa[nt][q >> 1].x;
+#pragma unroll
+                    for (int mt = 0; mt < 3; ++mt)
+                        acc[nt][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(cpart(cv[mt][q >> 1], q & 1), bv,
+                                                                            acc[nt][mt], 0, 0, 0);
+                }
+        } else {
+            bf16x8 ah[3], al[3];
+#pragma unroll
+            for (int mt = 0; mt < 3; ++mt)
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const float v = cpart(cv[mt][e >> 1], e & 1);
+                    const __bf16 hv = (__bf16)v;
+                    ah[mt][e] = hv;
+                    al[mt][e] = (__bf16)(v - (float)hv);
+                }
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                bf16x8 bh, bl;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const float v = (e & 1) ? xa[nt][e >> 1].y : xa[nt][e >> 1].x;
+                    const __bf16 hv = (__bf16)v;
+                    bh[e] = hv;
+                    bl[e] = (__bf16)(v - (float)hv);
+                }
+#pragma unroll
+                for (int mt = 0; mt < 3; ++mt) {
+                    acc[nt][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mt], bl, acc[nt][mt], 0, 0, 0);
+                    acc[nt][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[mt], bh, acc[nt][mt], 0, 0, 0);
+                    acc[nt][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mt], bh, acc[nt][mt], 0, 0, 0);
+                }
             }
+        }
     }
-    float2* o = U + zm.u_off + ((int64_t)b * nx + x) * 3;
-    o[0] = make_float2(acc[0], acc[1]);
-    o[1] = make_float2(acc[2], acc[3]);
-    o[2] = make_float2(acc[4], acc[5]);
+    // lane holds D[i = 4 g + r][x]: rows (m = 8 mt + 2 g + r / 2, part r % 2)
+#pragma unroll
+    for (int mt = 0; mt < 3; ++mt)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int m = 8 * mt + 2 * g + h;
+            const int b = b0 + m / 3;
+            if (b >= zm.my) continue;
+            float2* ub = U + zm.u_off + (int64_t)b * nx * 3 + (m % 3);
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                const int x = x0 + 16 * nt + j;
+                if (x < nx) ub[(int64_t)x * 3] = make_float2(acc[nt][mt][2 * h], acc[nt][mt][2 * h + 1]);
+            }
+        }
 }
 
 // Value of the main GEMM's B operand Bp[k][n] read from U row b = k >> 1.
@@ -733,10 +786,15 @@ int launch_gemm(const scw_exec* ex, float* out, hipStream_t st, int stage) {
         }
     }
     if (ex->my_max > 0 && (stage == 0 || stage == 2)) {
-        hipLaunchKernelGGL(u_kernel, dim3((unsigned)((ex->nx + 255) / 256), (unsigned)ex->my_max, (unsigned)ex->nz),
-                           dim3(256), 0, st, ex->d_zmeta.as<ZMeta>(), ex->d_regions.as<RegionDev>(),
-                           ex->d_C.as<float2>(), ex->c_elems, ex->d_X.as<float2>(), (int)ex->nx, ex->d_U.as<float2>(),
-                           0);
+        const dim3 g((unsigned)((ex->nx + 255) / 256), (unsigned)((ex->my_max + UB - 1) / UB), (unsigned)ex->nz);
+        if (ex->dtype == SCW_DTYPE_FP32)
+            hipLaunchKernelGGL(u_mfma<3>, g, dim3(256), 0, st, ex->d_zmeta.as<ZMeta>(), ex->d_regions.as<RegionDev>(),
+                               ex->d_C.as<float2>(), ex->c_elems, ex->d_X.as<float2>(), (int)ex->nx,
+                               ex->d_U.as<float2>(), 0);
+        else
+            hipLaunchKernelGGL(u_mfma<1>, g, dim3(256), 0, st, ex->d_zmeta.as<ZMeta>(), ex->d_regions.as<RegionDev>(),
+                               ex->d_C.as<float2>(), ex->c_elems, ex->d_X.as<float2>(), (int)ex->nx,
+                               ex->d_U.as<float2>(), 0);
         SCW_HIP(hipGetLastError());
     }
     if (stage != 0 && stage != 3) return SCW_OK;
