@@ -136,6 +136,29 @@ int scw_exec_set_launch(scw_exec* exec, int64_t coef_items, int64_t main_blocks)
 int scw_exec_graph_capture(scw_exec* exec, void* out_device, void* stream);
 int scw_exec_graph_launch(scw_exec* exec, void* stream);
 
+/* ---- On-device post-processing of fields (sCWatterView.cpp:194-352) -----
+ * Device buffers, asynchronous on `stream`, no allocation.  Results are
+ * bit-identical to the reference's float arithmetic.
+ * scw_post_scalar: EvaluateScalarSlices (sCWatterView.cpp:299-352).  field:
+ *   complex64 [n][3]; scalar: complex64 [n] = E[mode] (0 x, 1 y, 2 z) or
+ *   mode 3: sum_c E_c conj(E_c) (the intensity, imaginary part 0). */
+int scw_post_scalar(const void* field_device, size_t n, int mode, void* scalar_device, void* stream);
+/* scw_post_minmax: CalculateMinMax (sCWatterView.cpp:264-296).  minmax: float
+ * [4] on the device = (min Re, max Re, min Im, max Im) of scalar[0..n); with
+ * accumulate != 0 the previous contents are included, so several slices give
+ * one range.  NaN entries are skipped (the reference skips them except at
+ * index 0); no finite entry gives NaN. */
+int scw_post_minmax(const void* scalar_device, size_t n, float* minmax_device, int accumulate, void* stream);
+/* scw_post_color: EvaluateColorValue / EvaluateColorSlices
+ * (sCWatterView.cpp:194-262).  lowhigh: float[4] on the device = (real_low,
+ * real_high, imag_low, imag_high), the layout scw_post_minmax writes; rgb:
+ * uint8 [n][3] = 255 (clamp((Re - real_low) / (real_high - real_low))
+ * real_color + clamp((Im - imag_low) / (imag_high - imag_low)) imag_color),
+ * truncated; sums past 255 saturate (the reference's conversion is
+ * undefined there). */
+int scw_post_color(const void* scalar_device, size_t n, const float* lowhigh_device, const float real_color[3],
+                   const float imag_color[3], int show_real, int show_imag, void* rgb_device, void* stream);
+
 /* ---- .cw files (CoupledWaveStructure.h:7-134) --------------------------- */
 int scw_cw_load(const char* path, scw_cw** cw);
 int scw_cw_save(const scw_cw* cw, const char* path);
