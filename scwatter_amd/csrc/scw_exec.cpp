@@ -150,6 +150,12 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
         RegionDev& d = rd[v];
         d.ellA = P->d_ellA.as<float2>() + R.ell_off;
         d.arow = (int32_t)R.arow;
+        d.a24 = nullptr;
+        if (is16 && R.dense && R.width > 0) {  // the 16-bit mode contraction reads 24-bit amplitudes
+            int st2 = build_a24(const_cast<scw_plan*>(P));
+            if (st2 != SCW_OK) return st2;
+            d.a24 = P->d_a24.as<uint8_t>() + P->a24_off[r];
+        }
         d.ellC = R.ellc_off >= 0 ? P->d_ellC.as<int32_t>() + R.ellc_off : nullptr;
         d.cells = R.mx * R.my;
         d.width = R.width;

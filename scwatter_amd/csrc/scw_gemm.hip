@@ -25,6 +25,8 @@
 // the exact lane order of the MFMA B operand, so each is one coalesced load.
 #include "scw_internal.hpp"
 
+#include <type_traits>
+
 namespace scw {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -44,6 +46,9 @@ __device__ inline float2 stream_load(const float2* p) {
     const floatx2 v = __builtin_nontemporal_load((global_floatx2_ptr)(p));
     return make_float2(v[0], v[1]);
 }
+typedef uint32_t uint32x2 __attribute__((ext_vector_type(2)));
+typedef const __attribute__((address_space(1))) uint32x2* global_uint32x2_ptr;
+__device__ inline uint32x2 stream_load(const uint32x2* p) { return __builtin_nontemporal_load((global_uint32x2_ptr)(p)); }
 
 // ---------------------------------------------------------------------------
 // C_z[a,b,comp] partial sums over a term slice, for up to 8 z of one region.
@@ -618,6 +623,90 @@ static void launch_main(const scw_exec* ex, dim3 grid, hipStream_t st, int ytps,
 }
 
 // ---------------------------------------------------------------------------
+// 24-bit amplitudes for the 16-bit dtypes: each FP32 value rounded (to
+// nearest even) to its top 24 bits, i.e. a float with a 16-bit significand,
+// which splits exactly into bf16 hi (its top 16 bits) + bf16 lo (the rest).
+// Group q of a dense region holds modes 4q .. 4q + 3 of every column m:
+// record (q, m) = 24 bytes = [mode][re, im] x 3 little-endian bytes, stored
+// as three 8-byte planes [q][plane][arow] so that a load of one plane by 16
+// lanes (16 columns) reads 128 contiguous bytes.  Modes past the width and
+// columns past 3 cells are zero.
+// ---------------------------------------------------------------------------
+__device__ inline uint32_t top24(float v) {
+    const uint32_t u = __float_as_uint(v);
+    return (u + 0x7fu + ((u >> 8) & 1u)) >> 8;  // finite inputs (planned amplitudes stay below 1e30)
+}
+
+__global__ __launch_bounds__(256) void a24_encode(const float2* __restrict__ A, int64_t arow, int width, int mcols,
+                                                  int64_t groups, uint32_t* __restrict__ out) {
+    const int64_t n = groups * arow;
+    for (int64_t i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int64_t q = i / arow, m = i - q * arow;
+        uint32_t v[8];
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            const int64_t c = 4 * q + h;
+            const float2 a = (c < width && m < mcols) ? A[c * arow + m] : make_float2(0.0f, 0.0f);
+            v[2 * h] = top24(a.x);
+            v[2 * h + 1] = top24(a.y);
+        }
+        uint32_t w[6];
+        w[0] = v[0] | (v[1] << 24);
+        w[1] = (v[1] >> 8) | (v[2] << 16);
+        w[2] = (v[2] >> 16) | (v[3] << 8);
+        w[3] = v[4] | (v[5] << 24);
+        w[4] = (v[5] >> 8) | (v[6] << 16);
+        w[5] = (v[6] >> 16) | (v[7] << 8);
+#pragma unroll
+        for (int pl = 0; pl < 3; ++pl) {
+            uint32_t* o = out + ((q * 3 + pl) * arow + m) * 2;
+            o[0] = w[2 * pl];
+            o[1] = w[2 * pl + 1];
+        }
+    }
+}
+
+// The float (as bits) of 24-bit value k (0..7) of a record w[0..5].
+__device__ inline uint32_t a24_bits(const uint32_t (&w)[6], int k) {
+    const int b = 3 * (k & 3), o = (k >> 2) * 3;  // byte offset within the half record, word offset
+    switch (b) {
+        case 0: return w[o] << 8;
+        case 3: return __builtin_amdgcn_alignbit(w[o + 1], w[o], 16) & 0xffffff00u;
+        case 6: return __builtin_amdgcn_alignbit(w[o + 2], w[o + 1], 8) & 0xffffff00u;
+        default: return w[o + 2] & 0xffffff00u;
+    }
+}
+
+int build_a24(scw_plan* P) {
+    if (!P->a24_off.empty()) return SCW_OK;
+    std::vector<int64_t> off(P->regions.size(), -1);
+    int64_t bytes = 0;
+    for (size_t r = 0; r < P->regions.size(); ++r) {
+        const RegionPlan& R = P->regions[r];
+        if (!R.separable || !R.dense || R.width == 0) continue;
+        off[r] = bytes;
+        const int64_t groups = 4LL * ((R.width + 15) / 16);
+        bytes += groups * R.arow * 24;
+    }
+    int st;
+    if ((st = P->d_a24.alloc((size_t)std::max<int64_t>(bytes, 4))) != SCW_OK) return st;
+    for (size_t r = 0; r < P->regions.size(); ++r) {
+        if (off[r] < 0) continue;
+        const RegionPlan& R = P->regions[r];
+        const int64_t groups = 4LL * ((R.width + 15) / 16);
+        const int64_t n = groups * R.arow;
+        const dim3 g((unsigned)std::min<int64_t>((n + 255) / 256, 65536));
+        hipLaunchKernelGGL(a24_encode, g, dim3(256), 0, nullptr, P->d_ellA.as<float2>() + R.ell_off, R.arow,
+                           R.width, (int)(3 * (int64_t)R.mx * R.my), groups,
+                           reinterpret_cast<uint32_t*>(P->d_a24.as<uint8_t>() + off[r]));
+        SCW_HIP(hipGetLastError());
+    }
+    SCW_HIP(hipDeviceSynchronize());
+    P->a24_off = off;
+    return SCW_OK;
+}
+
+// ---------------------------------------------------------------------------
 // MFMA mode contraction for dense eigenmode regions (every cell holds every
 // kz; the term index is the kz index c):
 //   C_z[m] = sum_c Z_z[c] A[c][m],   m = complex column (cell, comp)
@@ -650,19 +739,37 @@ __device__ inline void modal_body(const ModalItem& I, const RegionDev& R, const 
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) acc[nt][mt] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 
-    // Raw operands of one 16-mode step.  Loads are unconditional: modes past
-    // the width re-read the last mode (their Z' entries are zero) and columns
-    // past the table re-read the last column (not stored).
-    typedef float2 RawA[4][4];
+    // Raw operands of one 16-mode step.  Loads are unconditional: FP32 modes
+    // past the width re-read the last mode (their Z' entries are zero), the
+    // 24-bit table is zero-padded to whole steps, and columns past the table
+    // re-read the last column (not stored).  FP32 (MODE 3) reads the float2
+    // amplitudes; the 16-bit modes read the 24-bit table: lane (j, g) of step
+    // t takes record (group 4 t + g, column), i.e. modes 16 t + 4 g .. + 3.
+    typedef float2 RawA[4][4];      // MODE 3: [nt][h] float2
+    typedef uint32_t RawA24[4][6];  // 16-bit modes: [nt] 24-byte record (3 planes of 8 bytes)
+    typedef typename std::conditional<MODE == 3, RawA, RawA24>::type RawT;
     typedef uint32x4 RawZ[MT][2];
     const uint32_t* zb = zf + I.zf_off + lane * 8;
-    auto load = [&](RawA& ra, RawZ& rz, int t) {
+    auto load = [&](RawT& ra, RawZ& rz, int t) {
+        if constexpr (MODE == 3) {
 #pragma unroll
-        for (int h = 0; h < 4; ++h) {
-            const int c = min(16 * t + (MODE == 3 ? g + 4 * h : 4 * g + h), W - 1);
-            const float2* row = A + (int64_t)c * R.arow;
+            for (int h = 0; h < 4; ++h) {
+                const int c = min(16 * t + g + 4 * h, W - 1);
+                const float2* row = A + (int64_t)c * R.arow;
 #pragma unroll
-            for (int nt = 0; nt < 4; ++nt) ra[nt][h] = MT >= 4 ? stream_load(row + col[nt]) : row[col[nt]];
+                for (int nt = 0; nt < 4; ++nt) ra[nt][h] = MT >= 4 ? stream_load(row + col[nt]) : row[col[nt]];
+            }
+        } else {
+            const uint32x2* grp = reinterpret_cast<const uint32x2*>(R.a24) + (int64_t)(4 * t + g) * 3 * R.arow;
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const uint32x2* p = grp + (int64_t)q * R.arow + col[nt];
+                    const uint32x2 v = MT >= 4 ? stream_load(p) : *p;
+                    ra[nt][2 * q] = v[0];
+                    ra[nt][2 * q + 1] = v[1];
+                }
         }
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
@@ -671,7 +778,7 @@ __device__ inline void modal_body(const ModalItem& I, const RegionDev& R, const 
             if constexpr (MODE == 1 || MODE == 3) rz[mt][1] = *reinterpret_cast<const uint32x4*>(p + 4);
         }
     };
-    auto step = [&](const RawA& ra, const RawZ& rz) {
+    auto step = [&](const RawT& ra, const RawZ& rz) {
         if constexpr (MODE == 3) {
 #pragma unroll
             for (int q = 0; q < 8; ++q)
@@ -693,14 +800,16 @@ __device__ inline void modal_body(const ModalItem& I, const RegionDev& R, const 
             for (int nt = 0; nt < 4; ++nt) {
                 vec bh, bl;
 #pragma unroll
-                for (int h = 0; h < 4; ++h) {
-                    const float2 u = ra[nt][h];
-                    const elem h0 = (elem)u.x, h1 = (elem)u.y;
-                    bh[2 * h] = h0;
-                    bh[2 * h + 1] = h1;
+                for (int e = 0; e < 8; ++e) {  // e = 2 h + (re|im)
+                    const uint32_t bits = a24_bits(ra[nt], e);
+                    const float v = __uint_as_float(bits);
                     if constexpr (MODE == 1) {
-                        bl[2 * h] = (elem)(u.x - (float)h0);
-                        bl[2 * h + 1] = (elem)(u.y - (float)h1);
+                        // exact split: hi = the top 16 bits, lo = the remaining 8 significant bits
+                        const float hv = __uint_as_float(bits & 0xffff0000u);
+                        bh[e] = __builtin_bit_cast(elem, (uint16_t)(bits >> 16));
+                        bl[e] = (elem)(v - hv);
+                    } else {
+                        bh[e] = (elem)v;
                     }
                 }
 #pragma unroll
@@ -719,7 +828,7 @@ __device__ inline void modal_body(const ModalItem& I, const RegionDev& R, const 
     // Two named buffers, the next step's loads in flight while the current
     // step computes; prefetch indices are clamped to the last step.
     const int t0 = I.c0 >> 4, t1 = I.c1 >> 4;
-    RawA a0, a1;
+    RawT a0, a1;
     RawZ z0, z1;
     load(a0, z0, t0);
     for (int t = t0;;) {
