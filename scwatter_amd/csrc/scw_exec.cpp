@@ -327,8 +327,11 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
             r0 = r1;
             continue;
         }
+        // terms once per run: 24 bytes (3 complex FP32), or 18 for the 16-bit
+        // mode contraction's 24-bit amplitudes; plus the Z rows
+        const double term_bytes = runs[r0].modal && is16 ? 18.0 : 24.0;
         for (size_t r = r0; r < r1; ++r)
-            b_coef += 24.0 * (double)d.cells * d.width + 8.0 * d.mc * (runs[r].z1 - runs[r].z0);
+            b_coef += term_bytes * (double)d.cells * d.width + 8.0 * d.mc * (runs[r].z1 - runs[r].z0);
         if (runs[r0].modal) {
             const RegionPlan& R = vreg[runs[r0].v].first->regions[vreg[runs[r0].v].second];
             const int T = (d.width + 15) / 16;
