@@ -25,6 +25,7 @@
 // the exact lane order of the MFMA B operand, so each is one coalesced load.
 #include "scw_internal.hpp"
 
+#include <mutex>
 #include <type_traits>
 
 namespace scw {
@@ -678,6 +679,9 @@ __device__ inline uint32_t a24_bits(const uint32_t (&w)[6], int k) {
 }
 
 int build_a24(scw_plan* P) {
+    // execs of one plan may be created from several threads
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lock(mu);
     if (!P->a24_off.empty()) return SCW_OK;
     std::vector<int64_t> off(P->regions.size(), -1);
     int64_t bytes = 0;
