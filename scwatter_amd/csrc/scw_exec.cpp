@@ -221,6 +221,11 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
         ex->kmax = kmax <= 16 ? 16 : kmax <= 32 ? 32 : kmax <= 64 ? 64 : 128;
     else
         ex->kmax = kmax <= 32 ? 32 : kmax <= 64 ? 64 : kmax <= 96 ? 96 : 128;
+    // The MFMA GEMM reads one z-slice of U (up to kmax / 2 rows of 6 nx
+    // floats, plus a lane overhang past the last column) with 32-bit offsets.
+    if (((double)ex->kmax / 2 * 6.0 * nx + 512.0) * 4.0 >= 2147483648.0)
+        return fail(SCW_ERR_UNSUPPORTED, "GEMM form: nx = %d is too large for %d distinct ky (U slice over 2 GiB)",
+                    nx, kmax / 2);
 
     // Runs: consecutive z of one virtual region.  Dense eigenmode regions
     // (every cell holds every kz, term index = kz index) contract on MFMA in

@@ -318,11 +318,26 @@ __global__ __launch_bounds__(256) void u_mfma(const ZMeta* __restrict__ zmeta, c
         }
 }
 
-// Value of the main GEMM's B operand Bp[k][n] read from U row b = k >> 1.
-__device__ inline float bp_value(const float* __restrict__ Ub, int64_t ncols, int k, int64_t n) {
-    const float2 u = *reinterpret_cast<const float2*>(Ub + (int64_t)(k >> 1) * ncols + (n & ~(int64_t)1));
+constexpr int kBufferRsrcWord3 = 0x00020000;  // raw dword buffer, gfx9 family
+constexpr int kStoreNT = 2;                   // non-temporal
+
+// The main GEMM's resident operand Bp[k][n], read from one z-slice of U (my
+// rows of ncols floats, under 2 GiB) through a raw buffer.  The hardware range
+// check returns zero for rows past my, which covers every k past the region's
+// K; a lane past the field (n >= ncols) reads part of the next row or zero,
+// which only reaches output columns that the range-checked stores drop.  So
+// the loads carry no conditions: a load under a lane-dependent branch is
+// waited for before the branch ends, which would serialize the fragment.
+__device__ inline __amdgpu_buffer_rsrc_t u_slice_rsrc(const float* Ub, int my, int64_t ncols) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(Ub), 0, (int)(my * ncols * 4), kBufferRsrcWord3);
+}
+__device__ inline float bp_buffer(__amdgpu_buffer_rsrc_t rs, int64_t ncols, int k, int64_t n) {
+    const int off = (int)(((int64_t)(k >> 1) * ncols + (n & ~(int64_t)1)) * 4);
+    const uint32x2 w = __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0);
+    const uint32_t wx = w.x, wy = w.y;
+    const float ux = __builtin_bit_cast(float, wx), uy = __builtin_bit_cast(float, wy);
     const bool odd = (n & 1) != 0;
-    return (k & 1) == 0 ? (odd ? u.y : u.x) : (odd ? u.x : -u.y);
+    return (k & 1) == 0 ? (odd ? uy : ux) : (odd ? ux : -uy);
 }
 
 // Epilogue of one 16-row y tile of a workgroup (256 output columns): every
@@ -339,9 +354,6 @@ constexpr int TROW = 256 + 4;
 // the hardware range check drops the lanes and rows outside the field, so
 // the stores carry no branches and the wait counters see the same number of
 // stores on every path.
-constexpr int kBufferRsrcWord3 = 0x00020000;  // raw dword buffer, gfx9 family
-constexpr int kStoreNT = 2;                   // non-temporal
-
 template <bool VEC>
 __device__ inline void row_store(float* orow, bool row_ok, int64_t n, int64_t ncols, const floatx4& v) {
     const uint32_t nrec = row_ok ? (uint32_t)(ncols * 4) : 0u;
@@ -411,7 +423,7 @@ __global__ __launch_bounds__(256) void main_f32(const ZMeta* __restrict__ zmeta,
     const int64_t ncols = 6LL * nx;
     const int64_t n0 = (int64_t)blockIdx.x * 256 + wv * 64;  // waves past ncols compute zeros
     const int li = lane & 15, lk = lane >> 4;
-    const float* Ub = reinterpret_cast<const float*>(U + zm.u_off);
+    const __amdgpu_buffer_rsrc_t urs = u_slice_rsrc(reinterpret_cast<const float*>(U + zm.u_off), zm.my, ncols);
     const int ytiles = (ny + 15) >> 4;
     const int yt0 = blockIdx.z * ytiles_per_split;
     const int yt1 = min(ytiles, yt0 + ytiles_per_split);
@@ -429,7 +441,7 @@ __global__ __launch_bounds__(256) void main_f32(const ZMeta* __restrict__ zmeta,
 #pragma unroll
         for (int s = 0; s < KMAX / 4; ++s) {
             const int k = 4 * s + lk;
-            uf[it][s] = ((s >> 2) < K16 && (k >> 1) < zm.my && n < ncols) ? bp_value(Ub, ncols, k, n) : 0.0f;
+            uf[it][s] = bp_buffer(urs, ncols, k, n);
         }
     }
 
@@ -525,7 +537,7 @@ __global__ __launch_bounds__(256) void main_16(const ZMeta* __restrict__ zmeta, 
     const int64_t ncols = 6LL * nx;
     const int64_t n0 = (int64_t)blockIdx.x * 256 + wv * 64;  // waves past ncols compute zeros
     const int li = lane & 15, lk = lane >> 4;
-    const float* Ub = reinterpret_cast<const float*>(U + zm.u_off);
+    const __amdgpu_buffer_rsrc_t urs = u_slice_rsrc(reinterpret_cast<const float*>(U + zm.u_off), zm.my, ncols);
     const int ytiles = (ny + 15) >> 4;
     const int yt0 = blockIdx.z * ytiles_per_split;
     const int yt1 = min(ytiles, yt0 + ytiles_per_split);
@@ -545,7 +557,7 @@ __global__ __launch_bounds__(256) void main_16(const ZMeta* __restrict__ zmeta, 
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
                 const int k = 32 * s + 8 * lk + e;
-                const float v = (s < K32 && (k >> 1) < zm.my && n < ncols) ? bp_value(Ub, ncols, k, n) : 0.0f;
+                const float v = bp_buffer(urs, ncols, k, n);
                 const elem h = (elem)v;
                 ahi[it][s][e] = h;
                 if constexpr (MODE == 1) alo[it][s][e] = (elem)(v - (float)h);
