@@ -449,6 +449,30 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
         for (int k = 0; k <= 4; ++k) ex->mclass_off[k] = 0;
         for (const ModalItem& it : ex->mitems) ++ex->mclass_off[cls(it) + 1];
         for (int k = 1; k <= 4; ++k) ex->mclass_off[k] += ex->mclass_off[k - 1];
+        // The 64-z class of the 16-bit modes runs one workgroup per four column
+        // chunks of one run and mode slot (the waves share the phase fragments
+        // in LDS): order it by (run, slot), keeping the column order, and pad
+        // every group to a multiple of four with items past the table (their
+        // waves read clamped columns and store nothing).
+        if (is16 && ex->mclass_off[4] > ex->mclass_off[3]) {
+            std::vector<ModalItem> out(ex->mitems.begin(), ex->mitems.begin() + ex->mclass_off[3]);
+            std::vector<ModalItem> cl(ex->mitems.begin() + ex->mclass_off[3], ex->mitems.end());
+            std::stable_sort(cl.begin(), cl.end(), [](const ModalItem& a, const ModalItem& b) {
+                return a.zf_off != b.zf_off ? a.zf_off < b.zf_off : a.slot < b.slot;
+            });
+            for (size_t i = 0; i < cl.size();) {
+                size_t k = i;
+                while (k < cl.size() && cl[k].zf_off == cl[i].zf_off && cl[k].slot == cl[i].slot) out.push_back(cl[k++]);
+                ModalItem pad = cl[k - 1];
+                for (size_t n = k - i; n % 4; ++n) {
+                    pad.m0 += 64;
+                    out.push_back(pad);
+                }
+                i = k;
+            }
+            ex->mitems.swap(out);
+            ex->mclass_off[4] = (int64_t)ex->mitems.size();
+        }
     }
     ex->stage_flops[1] = f_coef;
     ex->stage_flops[2] = f_u;
@@ -463,8 +487,9 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
     const int gx = (int)((6LL * nx + 255) / 256);
     // Each wave keeps its A-operand fragment (from U_z) in registers for all
     // of its y tiles, so split y across workgroups only as far as needed to
-    // give every CU a couple of workgroups.
-    const int64_t mtarget = ex->tune_main_blocks > 0 ? ex->tune_main_blocks : tunable("SCW_MAIN_BLOCKS", 512);
+    // give every CU several workgroups (about 1536 in all: enough that the
+    // fragment loads of starting workgroups overlap the stores of running ones).
+    const int64_t mtarget = ex->tune_main_blocks > 0 ? ex->tune_main_blocks : tunable("SCW_MAIN_BLOCKS", 1536);
     const int ysplit =
         (int)std::min<int64_t>(ytiles, std::max<int64_t>(1, (mtarget + (int64_t)gx * nz - 1) / ((int64_t)gx * nz)));
     ex->ytiles_per_split = (ytiles + ysplit - 1) / ysplit;

@@ -739,6 +739,28 @@ int build_a24(scw_plan* P) {
 //              <-> (c = g + 4 (q / 2), part q % 2)
 // Either way lane (j, g) reads the float2 A[c][m0 + 16 nt + j] of 4 modes.
 // ---------------------------------------------------------------------------
+// Partial sums of a mode-contraction item into its slot of C: lane holds
+// D[i = 4 g + r][m0 + 16 nt + j]: rows 4g..4g+3 are (z, re), (z, im),
+// (z + 1, re), (z + 1, im) with z = 8 mt + 2 g.
+template <int MT>
+__device__ inline void modal_store(const floatx4 (&acc)[4][MT], const ModalItem& I, const ZMeta* __restrict__ zmeta,
+                                   int64_t c_elems, float2* __restrict__ C, int mcols) {
+    const int lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int zi = 8 * mt + 2 * g + h;
+            if (zi >= I.nz) continue;
+            float2* o = C + (int64_t)I.slot * c_elems + zmeta[I.z0 + zi].c_off;
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                const int m = I.m0 + 16 * nt + j;
+                if (m < mcols) o[m] = make_float2(acc[nt][mt][2 * h], acc[nt][mt][2 * h + 1]);
+            }
+        }
+}
+
 template <int MODE, int MT>
 __device__ inline void modal_body(const ModalItem& I, const RegionDev& R, const ZMeta* __restrict__ zmeta,
                                   const uint32_t* __restrict__ zf, int64_t c_elems, float2* __restrict__ C) {
@@ -855,21 +877,128 @@ __device__ inline void modal_body(const ModalItem& I, const RegionDev& R, const 
         step(a1, z1);
         if (++t == t1) break;
     }
-    // lane holds D[i = 4 g + r][m0 + 16 nt + j]: rows 4g..4g+3 are (z, re),
-    // (z, im), (z + 1, re), (z + 1, im) with z = 8 mt + 2 g.
+    modal_store<MT>(acc, I, zmeta, c_elems, C, mcols);
+}
+
+// Runs of 64 z (MT = 8 row tiles) with the 16-bit modes.  All four waves of
+// a workgroup hold items of the same run and mode slot (the host pads each
+// run's column chunks to whole workgroups with items past the table, which
+// store nothing), so they need the same phase fragments: each step's 16 KB of
+// fragments is loaded once per workgroup, a quarter per wave, into
+// double-buffered LDS and read from there by every wave.  That takes three
+// quarters of the fragment traffic off the vector-memory return path, which
+// the fragments would otherwise occupy longer than the amplitudes.  One
+// barrier per step: a buffer is refilled only after every wave has passed the
+// barrier that follows its last read.  Amplitude loads run two steps ahead
+// (each step's raw records are decoded once into hi/lo operands for all four
+// column tiles, and their registers take the load of the step after next).
+template <int MODE>
+__device__ inline void modal_body_wg(const ModalItem& I, const ModalItem& L, const RegionDev& R,
+                                     const ZMeta* __restrict__ zmeta, const uint32_t* __restrict__ zf,
+                                     int64_t c_elems, float2* __restrict__ C) {
+    static_assert(MODE != 3, "16-bit modes");
+    constexpr int MT = 8, NH = MODE == 1 ? 2 : 1;  // fragment halves: high (and low)
+    typedef typename Op16<MODE>::vec vec;
+    typedef typename Op16<MODE>::elem elem;
+    __shared__ uint32x4 zs[2][MT][NH][64];
+    const int lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4;
+    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int mcols = 3 * R.cells;
+    int col[4];
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
+    for (int nt = 0; nt < 4; ++nt) col[nt] = min(I.m0 + 16 * nt + j, mcols - 1);
+    floatx4 acc[4][MT];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int zi = 8 * mt + 2 * g + h;
-            if (zi >= I.nz) continue;
-            float2* o = C + (int64_t)I.slot * c_elems + zmeta[I.z0 + zi].c_off;
+    for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) acc[nt][mt] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    typedef uint32_t RawA24[4][6];
+    // Unconditional loads, as in modal_body: the table is zero-padded to whole
+    // steps, columns past the table re-read the last column, and prefetches
+    // past the last step re-read it.
+    auto loadA = [&](RawA24& ra, int t) {
+        const uint32x2* grp = reinterpret_cast<const uint32x2*>(R.a24) + (int64_t)(4 * t + g) * 3 * R.arow;
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const uint32x2 v = stream_load(grp + (int64_t)q * R.arow + col[nt]);
+                ra[nt][2 * q] = v[0];
+                ra[nt][2 * q + 1] = v[1];
+            }
+    };
+    // This wave's quarter of a step's fragments: row tiles 2 wv, 2 wv + 1.
+    // The steps and fragments are the workgroup leader's L (the same run and
+    // slot as this wave's item I; I supplies the columns).
+    typedef uint32x4 Stage[2][NH];
+    const uint32_t* zb = zf + L.zf_off + lane * 8;
+    auto loadZ = [&](Stage& st, int t) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const uint32_t* p = zb + ((int64_t)(2 * wv + u) * L.T + t) * 512;
+#pragma unroll
+            for (int h = 0; h < NH; ++h) st[u][h] = *reinterpret_cast<const uint32x4*>(p + 4 * h);
+        }
+    };
+    auto storeZ = [&](const Stage& st, int buf) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int h = 0; h < NH; ++h) zs[buf][2 * wv + u][h][lane] = st[u][h];
+    };
+    const int t0 = L.c0 >> 4, t1 = L.c1 >> 4;
+    Stage stg;
+    loadZ(stg, t0);
+    RawA24 a0, a1;
+    loadA(a0, t0);
+    loadA(a1, min(t0 + 1, t1 - 1));
+    storeZ(stg, 0);
+    __syncthreads();
+    auto step = [&](RawA24& ra, int t, int buf) {
+        vec bh[4], bl[4];
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {  // e = 2 h + (re|im)
+                const uint32_t bits = a24_bits(ra[nt], e);
+                if constexpr (MODE == 1) {
+                    // exact split: hi = the top 16 bits, lo = the remaining 8 significant bits
+                    const float v = __uint_as_float(bits), hv = __uint_as_float(bits & 0xffff0000u);
+                    bh[nt][e] = __builtin_bit_cast(elem, (uint16_t)(bits >> 16));
+                    bl[nt][e] = (elem)(v - hv);
+                } else {
+                    bh[nt][e] = (elem)__uint_as_float(bits);
+                }
+            }
+        // next step's fragments first, then the amplitudes of the step after
+        // next: the fragments are waited for at the end of this step, the
+        // amplitudes only two steps later
+        loadZ(stg, min(t + 1, t1 - 1));
+        loadA(ra, min(t + 2, t1 - 1));
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            const vec zh = __builtin_bit_cast(vec, zs[buf][mt][0][lane]);
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt) {
-                const int m = I.m0 + 16 * nt + j;
-                if (m < mcols) o[m] = make_float2(acc[nt][mt][2 * h], acc[nt][mt][2 * h + 1]);
+                if constexpr (MODE == 1) {
+                    const vec zl = __builtin_bit_cast(vec, zs[buf][mt][NH - 1][lane]);
+                    acc[nt][mt] = mfma16<MODE>(zh, bl[nt], acc[nt][mt]);
+                    acc[nt][mt] = mfma16<MODE>(zl, bh[nt], acc[nt][mt]);
+                }
+                acc[nt][mt] = mfma16<MODE>(zh, bh[nt], acc[nt][mt]);
             }
         }
+        storeZ(stg, buf ^ 1);
+        __syncthreads();
+    };
+    for (int t = t0;;) {
+        step(a0, t, 0);
+        if (++t == t1) break;
+        step(a1, t, 1);
+        if (++t == t1) break;
+    }
+    modal_store<MT>(acc, I, zmeta, c_elems, C, mcols);
 }
 
 template <int MODE, int MT>
@@ -881,6 +1010,22 @@ __global__ __launch_bounds__(256) void modal_kernel(const ModalItem* __restrict_
     if (it >= nitems) return;
     const ModalItem I = items[it];
     modal_body<MODE, MT>(I, regions[I.region], zmeta, zf, c_elems, C);
+}
+
+// The 64-z class of the 16-bit modes (modal_body_wg), two waves per SIMD.
+template <int MODE>
+__global__ __launch_bounds__(256, 2) void modal_wg_kernel(const ModalItem* __restrict__ items, int nitems,
+                                                          const RegionDev* __restrict__ regions,
+                                                          const ZMeta* __restrict__ zmeta,
+                                                          const uint32_t* __restrict__ zf, int64_t c_elems,
+                                                          float2* __restrict__ C) {
+    // Whole workgroups only (the host pads this class to them), and the step
+    // range comes from the workgroup's first item, so every wave passes the
+    // same barriers.
+    if ((int)blockIdx.x * 4 + 3 >= nitems) return;
+    const int it = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+    const ModalItem I = items[it];
+    modal_body_wg<MODE>(I, items[blockIdx.x * 4], regions[I.region], zmeta, zf, c_elems, C);
 }
 
 template <int MODE>
@@ -898,7 +1043,12 @@ static void launch_modal(const scw_exec* ex, hipStream_t st) {
             case 0: hipLaunchKernelGGL((modal_kernel<MODE, 1>), g, dim3(256), 0, st, mi + off, (int)n, rg, zm, zf, ex->c_elems, C); break;
             case 1: hipLaunchKernelGGL((modal_kernel<MODE, 2>), g, dim3(256), 0, st, mi + off, (int)n, rg, zm, zf, ex->c_elems, C); break;
             case 2: hipLaunchKernelGGL((modal_kernel<MODE, 4>), g, dim3(256), 0, st, mi + off, (int)n, rg, zm, zf, ex->c_elems, C); break;
-            default: hipLaunchKernelGGL((modal_kernel<MODE, 8>), g, dim3(256), 0, st, mi + off, (int)n, rg, zm, zf, ex->c_elems, C); break;
+            default:
+                if constexpr (MODE == 3)
+                    hipLaunchKernelGGL((modal_kernel<MODE, 8>), g, dim3(256), 0, st, mi + off, (int)n, rg, zm, zf, ex->c_elems, C);
+                else
+                    hipLaunchKernelGGL((modal_wg_kernel<MODE>), g, dim3(256), 0, st, mi + off, (int)n, rg, zm, zf, ex->c_elems, C);
+                break;
         }
     }
 }

@@ -132,7 +132,8 @@ def slab_dense():
     cw = configs.tile_sample_structure(M=7, Z=1, size=(12.0, 12.0, 2.0), lam=0.7, seed=3, incidence_deg=12.0)
     t = cw.unpack()
     # 75 z inside the slab: one stretch of the dense region, split into runs
-    # of 25 (4 row tiles of the MFMA mode contraction); 5 z outside it
+    # of 38 and 37 z (5 row tiles, in the 8-tile class of the MFMA mode
+    # contraction, with partly filled tiles); 5 z outside it
     zs = np.concatenate([configs.axis(-0.95, 0.95, 75), configs.axis(1.05, 1.6, 5)])
     xs, ys = configs.axis(-6, 6, 18), configs.axis(-6, 6, 14)
     ref = oracle.eval_grid_f64(t.W, t.z_layers, t.begin, t.end, xs, ys, zs)
@@ -142,7 +143,7 @@ def slab_dense():
 @pytest.mark.parametrize("dtype", ["fp32", "bf16x3", "bf16", "fp16"])
 def test_dense_slab_mode_contraction(slab_dense, dtype, monkeypatch):
     """Long z stretches through a dense eigenmode region: the MFMA mode
-    contraction (runs of up to 32 z) and the VALU term contraction (forced
+    contraction (runs of up to 64 z) and the VALU term contraction (forced
     with SCW_MODAL_MIN_RUN) both match the oracle."""
     plan, g, ref = slab_dense
     info = plan.region_info(1)  # every cell holds every kz: dense
