@@ -107,8 +107,11 @@ def stage_kernels(stage, dt):
     """Kernel-name patterns of one GEMM-form stage in one dtype."""
     mode = {"fp32": 3, "bf16": 0, "bf16x3": 1, "fp16": 2}[dt]
     main = r"scw::main_f32<\d+, (true|false)>" if dt == "fp32" else r"scw::main_16<\d+, (true|false), %d>" % mode
-    return {"coef": [r"scw::coef_kernel", r"scw::modal_kernel<%d, \d>" % mode],
-            "xcontract": [r"scw::u_kernel"], "main": [main]}[stage]
+    coef = [r"scw::coef_kernel", r"scw::modal_kernel<%d, \d>" % mode]
+    if dt != "fp32":
+        coef.append(r"scw::modal_wg_kernel<%d>" % mode)
+    xc = r"scw::u_mfma<%d>" % (3 if dt == "fp32" else 1)
+    return {"coef": coef, "xcontract": [xc], "main": [main]}[stage]
 
 
 def roofline(flops, nbytes, ms, peak_flops):
