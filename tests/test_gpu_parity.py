@@ -281,11 +281,20 @@ def test_errors_are_reported():
         Plan(WaveTables(W, np.zeros(0), np.array([0], np.int32), np.array([n], np.int32)), device=99)
 
 
-@pytest.mark.parametrize("dtype", ["bf16x3", "fp32"])
-def test_batch_of_uneven_instances(dtype):
-    """One batch run over instances of different order counts equals each instance against the oracle."""
+@pytest.mark.parametrize("dtype,runs", [("bf16x3", "short"), ("fp32", "short"), ("bf16x3", "long"),
+                                        ("bf16", "long"), ("fp32", "long")])
+def test_batch_of_uneven_instances(dtype, runs):
+    """One batch run over instances of different order counts equals each
+    instance against the oracle.  Short runs (9-12 z) take the 2-tile class of
+    the MFMA mode contraction; long runs (33-64 z) the 8-tile class, whose
+    16-bit kernel runs one workgroup per four column chunks of a run and mode
+    slot (the instances have 1, 2 and 3 chunks of 64 columns, so every group
+    is padded with items past the table)."""
     from scwatter_amd import BatchExec, hip
-    specs = [(3, 11, 0.6, 5.0), (5, 12, 0.9, 20.0), (7, 9, 0.45, 35.0)]
+    if runs == "short":
+        specs = [(3, 11, 0.6, 5.0), (5, 12, 0.9, 20.0), (7, 9, 0.45, 35.0)]
+    else:
+        specs = [(4, 64, 0.6, 5.0), (6, 47, 0.9, 20.0), (8, 33, 0.45, 35.0)]
     xs, ys = configs.axis(-6, 6, 20), configs.axis(-6, 6, 16)
     plans, tables, zs_list = [], [], []
     for M, nz, lam, ang in specs:
