@@ -94,7 +94,8 @@ int scw_plan_layers(const scw_plan* plan, int* layers);
 int scw_plan_region_info(const scw_plan* plan, int region, int64_t* n_waves, int32_t* mx, int32_t* my,
                          int32_t* mc, int64_t* n_terms, int32_t* width, int32_t* separable);
 /* Device memory held by the plan; grows once when the first exec with a
- * 16-bit dtype adds the 24-bit amplitude table of its dense regions. */
+ * 16-bit dtype adds the block floating-point amplitude table of its dense
+ * regions. */
 int scw_plan_device_bytes(const scw_plan* plan, size_t* bytes);
 
 /* ---- grid execution (replaces gpu_cw_evaluate / kernel_{xy,xz,yz}) ------ */

@@ -150,11 +150,12 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
         RegionDev& d = rd[v];
         d.ellA = P->d_ellA.as<float2>() + R.ell_off;
         d.arow = (int32_t)R.arow;
-        d.a24 = nullptr;
-        if (is16 && R.dense && R.width > 0) {  // the 16-bit mode contraction reads 24-bit amplitudes
-            int st2 = build_a24(const_cast<scw_plan*>(P));
+        d.bfp = d.bfp_exp = nullptr;
+        if (is16 && R.dense && R.width > 0) {  // the 16-bit mode contraction reads block floating point
+            int st2 = build_bfp(const_cast<scw_plan*>(P));
             if (st2 != SCW_OK) return st2;
-            d.a24 = P->d_a24.as<uint8_t>() + P->a24_off[r];
+            d.bfp = P->d_bfp.as<uint8_t>() + P->bfp_off[r];
+            d.bfp_exp = d.bfp + 4LL * ((R.width + 15) / 16) * R.arow * 16;
         }
         d.ellC = R.ellc_off >= 0 ? P->d_ellC.as<int32_t>() + R.ellc_off : nullptr;
         d.cells = R.mx * R.my;
@@ -332,9 +333,10 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
             r0 = r1;
             continue;
         }
-        // terms once per run: 24 bytes (3 complex FP32), or 18 for the 16-bit
-        // mode contraction's 24-bit amplitudes; plus the Z rows
-        const double term_bytes = runs[r0].modal && is16 ? 18.0 : 24.0;
+        // terms once per run: 24 bytes (3 complex FP32), or 15 for the 16-bit
+        // mode contraction's block floating-point amplitudes (3 columns of 2
+        // 16-bit mantissas and an exponent byte); plus the Z rows
+        const double term_bytes = runs[r0].modal && is16 ? 15.0 : 24.0;
         for (size_t r = r0; r < r1; ++r)
             b_coef += term_bytes * (double)d.cells * d.width + 8.0 * d.mc * (runs[r].z1 - runs[r].z0);
         if (runs[r0].modal) {

@@ -50,6 +50,27 @@ __device__ inline float2 stream_load(const float2* p) {
 typedef uint32_t uint32x2 __attribute__((ext_vector_type(2)));
 typedef const __attribute__((address_space(1))) uint32x2* global_uint32x2_ptr;
 __device__ inline uint32x2 stream_load(const uint32x2* p) { return __builtin_nontemporal_load((global_uint32x2_ptr)(p)); }
+typedef const __attribute__((address_space(1))) uint32x4* global_uint32x4_ptr;
+__device__ inline uint32x4 stream_load(const uint32x4* p) { return __builtin_nontemporal_load((global_uint32x4_ptr)(p)); }
+typedef const __attribute__((address_space(1))) uint32_t* global_uint32_ptr;
+__device__ inline uint32_t stream_load(const uint32_t* p) { return __builtin_nontemporal_load((global_uint32_ptr)(p)); }
+
+// Ordinary load through a pointer read from a device struct (the region
+// tables).  The compiler cannot infer the address space of such a pointer and
+// emits a FLAT load, which counts against both the vector-memory and the LDS
+// wait counters; the first wait for anything then also waits for every
+// prefetch in flight.  The cast makes it a global load.
+template <typename T>
+__device__ inline T gload(const T* p) {
+    typedef const __attribute__((address_space(1))) T* global_ptr;
+    return *(global_ptr)(p);  // an address-space cast (reinterpret_cast refuses it)
+}
+// float2's copy constructor takes a generic reference, which would turn the
+// load back into a FLAT one: load the two floats as a vector instead.
+__device__ inline float2 gload(const float2* p) {
+    const floatx2 v = *(global_floatx2_ptr)(p);
+    return make_float2(v[0], v[1]);
+}
 
 // ---------------------------------------------------------------------------
 // C_z[a,b,comp] partial sums over a term slice, for up to 8 z of one region.
@@ -80,10 +101,10 @@ __device__ inline void coef_loop(float (&acc)[8][6], const float2* __restrict__ 
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const float2* p = Ab + (int64_t)(jj + u) * arow;
-            a[u][0] = p[0];
-            a[u][1] = p[1];
-            a[u][2] = p[2];
-            c[u] = DENSE ? jj + u : Cb[(int64_t)(jj + u) * stride];
+            a[u][0] = gload(p);
+            a[u][1] = gload(p + 1);
+            a[u][2] = gload(p + 2);
+            c[u] = DENSE ? jj + u : gload(Cb + (int64_t)(jj + u) * stride);
         }
     };
     auto mac = [&](const Group& a, const int (&c)[4]) {
@@ -120,8 +141,8 @@ __device__ inline void coef_loop(float (&acc)[8][6], const float2* __restrict__ 
     }
     for (; j < j1; ++j) {
         const float2* p = Ab + (int64_t)j * arow;
-        const float2 a0 = p[0], a1 = p[1], a2 = p[2];
-        const int c = DENSE ? j : Cb[(int64_t)j * stride];
+        const float2 a0 = gload(p), a1 = gload(p + 1), a2 = gload(p + 2);
+        const int c = DENSE ? j : gload(Cb + (int64_t)j * stride);
 #pragma unroll
         for (int zi = 0; zi < NZ; ++zi) cmac6(acc[zi], a0, a1, a2, Z[zrow[zi] + c]);
     }
@@ -636,65 +657,99 @@ static void launch_main(const scw_exec* ex, dim3 grid, hipStream_t st, int ytps,
 }
 
 // ---------------------------------------------------------------------------
-// 24-bit amplitudes for the 16-bit dtypes: each FP32 value rounded (to
-// nearest even) to its top 24 bits, i.e. a float with a 16-bit significand,
-// which splits exactly into bf16 hi (its top 16 bits) + bf16 lo (the rest).
-// Group q of a dense region holds modes 4q .. 4q + 3 of every column m:
-// record (q, m) = 24 bytes = [mode][re, im] x 3 little-endian bytes, stored
-// as three 8-byte planes [q][plane][arow] so that a load of one plane by 16
-// lanes (16 columns) reads 128 contiguous bytes.  Modes past the width and
-// columns past 3 cells are zero.
+// Block floating-point amplitudes for the 16-bit dtypes.  Group q of a dense
+// region holds modes 4q .. 4q + 3 of every column m; record (q, m) is the 8
+// values v[e], e = 2 h + (re|im) of mode 4q + h, as 16-bit signed mantissas,
+// the re/im pair of each mode sharing one exponent byte:
+// v[e] = mant[e] 2^(E[h] - 128), E[h] chosen so that the larger part has 15
+// significant bits.  Every value then has at most 15 significant bits and
+// splits exactly into bf16 hi (its top 16 bits) + bf16 lo (the rest); its
+// error is at most 2^-15 of its mode's |re| or |im| (half a step of the
+// 15-bit scale), so each term keeps its own relative precision whatever the
+// magnitudes of the other modes (an exponent shared across modes loses it
+// for weak modes whose phase factor is large: rel-L2 3.4e-4 on the 64-slab
+// batch).  Planes: mantissas [q][arow] x 16 bytes (a 16-lane load of 16
+// columns reads 256 contiguous bytes), then exponents [q][arow] x 4 bytes.
+// 20 bytes per record against 24 for FP32 values rounded to 24 bits.  Modes
+// past the width and columns past 3 cells are zero.
 // ---------------------------------------------------------------------------
-__device__ inline uint32_t top24(float v) {
-    const uint32_t u = __float_as_uint(v);
-    return (u + 0x7fu + ((u >> 8) & 1u)) >> 8;  // finite inputs (planned amplitudes stay below 1e30)
-}
+constexpr int kBfpBias = 128;
 
-__global__ __launch_bounds__(256) void a24_encode(const float2* __restrict__ A, int64_t arow, int width, int mcols,
-                                                  int64_t groups, uint32_t* __restrict__ out) {
+__global__ __launch_bounds__(256) void bfp_encode(const float2* __restrict__ A, int64_t arow, int width, int mcols,
+                                                  int64_t groups, uint32x4* __restrict__ mant,
+                                                  uint32_t* __restrict__ expo) {
     const int64_t n = groups * arow;
     for (int64_t i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
         const int64_t q = i / arow, m = i - q * arow;
-        uint32_t v[8];
+        uint32x4 w;
+        uint32_t ex = 0;
 #pragma unroll
         for (int h = 0; h < 4; ++h) {
             const int64_t c = 4 * q + h;
             const float2 a = (c < width && m < mcols) ? A[c * arow + m] : make_float2(0.0f, 0.0f);
-            v[2 * h] = top24(a.x);
-            v[2 * h + 1] = top24(a.y);
+            const float mx = fmaxf(fabsf(a.x), fabsf(a.y));
+            // scale 2^s with mx 2^-s in [2^14, 2^15) (one step up if it rounds
+            // to 2^15), kept a normal float; a zero mode takes s = 0
+            int s = 0;
+            if (mx > 0.0f) {
+                s = max(ilogbf(mx) - 14, -126);
+                if (rintf(ldexpf(mx, -s)) > 32767.0f) ++s;
+            }
+            const int re = (int)rintf(ldexpf(a.x, -s)), im = (int)rintf(ldexpf(a.y, -s));
+            w[h] = ((uint32_t)re & 0xffffu) | ((uint32_t)im << 16);
+            ex |= (uint32_t)(s + kBfpBias) << (8 * h);
         }
-        uint32_t w[6];
-        w[0] = v[0] | (v[1] << 24);
-        w[1] = (v[1] >> 8) | (v[2] << 16);
-        w[2] = (v[2] >> 16) | (v[3] << 8);
-        w[3] = v[4] | (v[5] << 24);
-        w[4] = (v[5] >> 8) | (v[6] << 16);
-        w[5] = (v[6] >> 16) | (v[7] << 8);
+        mant[i] = w;
+        expo[i] = ex;
+    }
+}
+
+// Value k (0..7) of a block floating-point record: mantissa words w, scale of
+// its mode.
+__device__ inline float bfp_value(const uint32_t (&w)[4], float scale, int k) {
+    const uint32_t x = w[k >> 1];
+    const int mant = (k & 1) ? ((int)x >> 16) : (int)(int16_t)(x & 0xffffu);
+    return (float)mant * scale;
+}
+// 2^(E - 128) from an exponent byte E (2 <= E <= 255 by construction).
+__device__ inline float bfp_scale(uint32_t e) { return __uint_as_float((e - (kBfpBias - 127)) << 23); }
+
+// One lane's record: the 4 mantissa words and the exponent word.
+typedef uint32_t RawBfp[5];
+template <bool NT>
+__device__ inline void bfp_load(RawBfp& r, const RegionDev& R, int64_t rec) {
+    const uint32x4* mp = reinterpret_cast<const uint32x4*>(R.bfp) + rec;
+    const uint32x4 w = NT ? stream_load(mp) : gload(mp);
 #pragma unroll
-        for (int pl = 0; pl < 3; ++pl) {
-            uint32_t* o = out + ((q * 3 + pl) * arow + m) * 2;
-            o[0] = w[2 * pl];
-            o[1] = w[2 * pl + 1];
+    for (int k = 0; k < 4; ++k) r[k] = w[k];
+    const uint32_t* ep = reinterpret_cast<const uint32_t*>(R.bfp_exp) + rec;
+    r[4] = NT ? stream_load(ep) : gload(ep);
+}
+// 16-bit MFMA operands of a record: MODE 1 the exact hi/lo split, MODE 0 / 2
+// the value rounded to bf16 / fp16.
+template <int MODE>
+__device__ inline void bfp_decode(const RawBfp& r, typename Op16<MODE>::vec& bh, typename Op16<MODE>::vec& bl) {
+    typedef typename Op16<MODE>::elem elem;
+    const uint32_t w[4] = {r[0], r[1], r[2], r[3]};
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {  // e = 2 h + (re|im)
+        const float v = bfp_value(w, bfp_scale((r[4] >> (8 * (e >> 1))) & 0xffu), e);
+        if constexpr (MODE == 1) {
+            // exact split: hi = the top 16 bits, lo = the remaining (at most 7) significant bits
+            const uint32_t bits = __float_as_uint(v);
+            bh[e] = __builtin_bit_cast(elem, (uint16_t)(bits >> 16));
+            bl[e] = (elem)(v - __uint_as_float(bits & 0xffff0000u));
+        } else {
+            bh[e] = (elem)v;
         }
     }
 }
 
-// The float (as bits) of 24-bit value k (0..7) of a record w[0..5].
-__device__ inline uint32_t a24_bits(const uint32_t (&w)[6], int k) {
-    const int b = 3 * (k & 3), o = (k >> 2) * 3;  // byte offset within the half record, word offset
-    switch (b) {
-        case 0: return w[o] << 8;
-        case 3: return __builtin_amdgcn_alignbit(w[o + 1], w[o], 16) & 0xffffff00u;
-        case 6: return __builtin_amdgcn_alignbit(w[o + 2], w[o + 1], 8) & 0xffffff00u;
-        default: return w[o + 2] & 0xffffff00u;
-    }
-}
-
-int build_a24(scw_plan* P) {
+int build_bfp(scw_plan* P) {
     // execs of one plan may be created from several threads
     static std::mutex mu;
     std::lock_guard<std::mutex> lock(mu);
-    if (!P->a24_off.empty()) return SCW_OK;
+    if (!P->bfp_off.empty()) return SCW_OK;
     std::vector<int64_t> off(P->regions.size(), -1);
     int64_t bytes = 0;
     for (size_t r = 0; r < P->regions.size(); ++r) {
@@ -702,23 +757,25 @@ int build_a24(scw_plan* P) {
         if (!R.separable || !R.dense || R.width == 0) continue;
         off[r] = bytes;
         const int64_t groups = 4LL * ((R.width + 15) / 16);
-        bytes += groups * R.arow * 24;
+        bytes += groups * R.arow * 20;
+        bytes = (bytes + 255) / 256 * 256;  // next region's mantissas 256-byte aligned
     }
     int st;
-    if ((st = P->d_a24.alloc((size_t)std::max<int64_t>(bytes, 4))) != SCW_OK) return st;
+    if ((st = P->d_bfp.alloc((size_t)std::max<int64_t>(bytes, 4))) != SCW_OK) return st;
     for (size_t r = 0; r < P->regions.size(); ++r) {
         if (off[r] < 0) continue;
         const RegionPlan& R = P->regions[r];
         const int64_t groups = 4LL * ((R.width + 15) / 16);
         const int64_t n = groups * R.arow;
+        uint8_t* base = P->d_bfp.as<uint8_t>() + off[r];
         const dim3 g((unsigned)std::min<int64_t>((n + 255) / 256, 65536));
-        hipLaunchKernelGGL(a24_encode, g, dim3(256), 0, nullptr, P->d_ellA.as<float2>() + R.ell_off, R.arow,
-                           R.width, (int)(3 * (int64_t)R.mx * R.my), groups,
-                           reinterpret_cast<uint32_t*>(P->d_a24.as<uint8_t>() + off[r]));
+        hipLaunchKernelGGL(bfp_encode, g, dim3(256), 0, nullptr, P->d_ellA.as<float2>() + R.ell_off, R.arow,
+                           R.width, (int)(3 * (int64_t)R.mx * R.my), groups, reinterpret_cast<uint32x4*>(base),
+                           reinterpret_cast<uint32_t*>(base + n * 16));
         SCW_HIP(hipGetLastError());
     }
     SCW_HIP(hipDeviceSynchronize());
-    P->a24_off = off;
+    P->bfp_off = off;
     return SCW_OK;
 }
 
@@ -779,13 +836,14 @@ __device__ inline void modal_body(const ModalItem& I, const RegionDev& R, const 
 
     // Raw operands of one 16-mode step.  Loads are unconditional: FP32 modes
     // past the width re-read the last mode (their Z' entries are zero), the
-    // 24-bit table is zero-padded to whole steps, and columns past the table
-    // re-read the last column (not stored).  FP32 (MODE 3) reads the float2
-    // amplitudes; the 16-bit modes read the 24-bit table: lane (j, g) of step
-    // t takes record (group 4 t + g, column), i.e. modes 16 t + 4 g .. + 3.
-    typedef float2 RawA[4][4];      // MODE 3: [nt][h] float2
-    typedef uint32_t RawA24[4][6];  // 16-bit modes: [nt] 24-byte record (3 planes of 8 bytes)
-    typedef typename std::conditional<MODE == 3, RawA, RawA24>::type RawT;
+    // block floating-point table is zero-padded to whole steps, and columns
+    // past the table re-read the last column (not stored).  FP32 (MODE 3)
+    // reads the float2 amplitudes; the 16-bit modes read the block
+    // floating-point table: lane (j, g) of step t takes record (group 4 t + g,
+    // column), i.e. modes 16 t + 4 g .. + 3.
+    typedef float2 RawA[4][4];   // MODE 3: [nt][h] float2
+    typedef RawBfp RawB[4];      // 16-bit modes: [nt] record
+    typedef typename std::conditional<MODE == 3, RawA, RawB>::type RawT;
     typedef uint32x4 RawZ[MT][2];
     const uint32_t* zb = zf + I.zf_off + lane * 8;
     auto load = [&](RawT& ra, RawZ& rz, int t) {
@@ -795,19 +853,12 @@ __device__ inline void modal_body(const ModalItem& I, const RegionDev& R, const 
                 const int c = min(16 * t + g + 4 * h, W - 1);
                 const float2* row = A + (int64_t)c * R.arow;
 #pragma unroll
-                for (int nt = 0; nt < 4; ++nt) ra[nt][h] = MT >= 4 ? stream_load(row + col[nt]) : row[col[nt]];
+                for (int nt = 0; nt < 4; ++nt) ra[nt][h] = MT >= 4 ? stream_load(row + col[nt]) : gload(row + col[nt]);
             }
         } else {
-            const uint32x2* grp = reinterpret_cast<const uint32x2*>(R.a24) + (int64_t)(4 * t + g) * 3 * R.arow;
+            const int64_t grp = (int64_t)(4 * t + g) * R.arow;
 #pragma unroll
-            for (int nt = 0; nt < 4; ++nt)
-#pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    const uint32x2* p = grp + (int64_t)q * R.arow + col[nt];
-                    const uint32x2 v = MT >= 4 ? stream_load(p) : *p;
-                    ra[nt][2 * q] = v[0];
-                    ra[nt][2 * q + 1] = v[1];
-                }
+            for (int nt = 0; nt < 4; ++nt) bfp_load<(MT >= 4)>(ra[nt], R, grp + col[nt]);
         }
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
@@ -833,23 +884,10 @@ __device__ inline void modal_body(const ModalItem& I, const RegionDev& R, const 
                 }
         } else {
             typedef typename Op16<MODE>::vec vec;
-            typedef typename Op16<MODE>::elem elem;
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt) {
                 vec bh, bl;
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {  // e = 2 h + (re|im)
-                    const uint32_t bits = a24_bits(ra[nt], e);
-                    const float v = __uint_as_float(bits);
-                    if constexpr (MODE == 1) {
-                        // exact split: hi = the top 16 bits, lo = the remaining 8 significant bits
-                        const float hv = __uint_as_float(bits & 0xffff0000u);
-                        bh[e] = __builtin_bit_cast(elem, (uint16_t)(bits >> 16));
-                        bl[e] = (elem)(v - hv);
-                    } else {
-                        bh[e] = (elem)v;
-                    }
-                }
+                bfp_decode<MODE>(ra[nt], bh, bl);
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt) {
                     const vec zh = __builtin_bit_cast(vec, rz[mt][0]);
@@ -899,7 +937,6 @@ __device__ inline void modal_body_wg(const ModalItem& I, const ModalItem& L, con
     static_assert(MODE != 3, "16-bit modes");
     constexpr int MT = 8, NH = MODE == 1 ? 2 : 1;  // fragment halves: high (and low)
     typedef typename Op16<MODE>::vec vec;
-    typedef typename Op16<MODE>::elem elem;
     __shared__ uint32x4 zs[2][MT][NH][64];
     const int lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4;
     const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
@@ -913,20 +950,14 @@ __device__ inline void modal_body_wg(const ModalItem& I, const ModalItem& L, con
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) acc[nt][mt] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 
-    typedef uint32_t RawA24[4][6];
+    typedef RawBfp RawB[4];
     // Unconditional loads, as in modal_body: the table is zero-padded to whole
     // steps, columns past the table re-read the last column, and prefetches
     // past the last step re-read it.
-    auto loadA = [&](RawA24& ra, int t) {
-        const uint32x2* grp = reinterpret_cast<const uint32x2*>(R.a24) + (int64_t)(4 * t + g) * 3 * R.arow;
+    auto loadA = [&](RawB& ra, int t) {
+        const int64_t grp = (int64_t)(4 * t + g) * R.arow;
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt)
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                const uint32x2 v = stream_load(grp + (int64_t)q * R.arow + col[nt]);
-                ra[nt][2 * q] = v[0];
-                ra[nt][2 * q + 1] = v[1];
-            }
+        for (int nt = 0; nt < 4; ++nt) bfp_load<true>(ra[nt], R, grp + col[nt]);
     };
     // This wave's quarter of a step's fragments: row tiles 2 wv, 2 wv + 1.
     // The steps and fragments are the workgroup leader's L (the same run and
@@ -950,32 +981,24 @@ __device__ inline void modal_body_wg(const ModalItem& I, const ModalItem& L, con
     const int t0 = L.c0 >> 4, t1 = L.c1 >> 4;
     Stage stg;
     loadZ(stg, t0);
-    RawA24 a0, a1;
+    RawB a0, a1;
     loadA(a0, t0);
     loadA(a1, min(t0 + 1, t1 - 1));
     storeZ(stg, 0);
     __syncthreads();
-    auto step = [&](RawA24& ra, int t, int buf) {
+    auto step = [&](RawB& ra, int t, int buf) {
         vec bh[4], bl[4];
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt)
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {  // e = 2 h + (re|im)
-                const uint32_t bits = a24_bits(ra[nt], e);
-                if constexpr (MODE == 1) {
-                    // exact split: hi = the top 16 bits, lo = the remaining 8 significant bits
-                    const float v = __uint_as_float(bits), hv = __uint_as_float(bits & 0xffff0000u);
-                    bh[nt][e] = __builtin_bit_cast(elem, (uint16_t)(bits >> 16));
-                    bl[nt][e] = (elem)(v - hv);
-                } else {
-                    bh[nt][e] = (elem)__uint_as_float(bits);
-                }
-            }
+        for (int nt = 0; nt < 4; ++nt) bfp_decode<MODE>(ra[nt], bh[nt], bl[nt]);
         // next step's fragments first, then the amplitudes of the step after
         // next: the fragments are waited for at the end of this step, the
         // amplitudes only two steps later
         loadZ(stg, min(t + 1, t1 - 1));
         loadA(ra, min(t + 2, t1 - 1));
+        // keep the loads ahead of the MFMAs: left to itself the scheduler
+        // sinks them to the end of the step, where the fragment wait before
+        // the barrier then exposes their whole latency
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
             const vec zh = __builtin_bit_cast(vec, zs[buf][mt][0][lane]);

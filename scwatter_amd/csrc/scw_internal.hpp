@@ -84,12 +84,13 @@ struct scw_plan {
     scw::DevBuf d_ellA;  // float2[...][width][arow]: [cells][3] E0 re-referenced to zref, padded rows
     scw::DevBuf d_ellC;  // int32[...][width][cells]      kz index (sparse regions only)
     int64_t ell_elems = 0;
-    // Dense-region amplitudes for the 16-bit GEMM dtypes, rounded to their top
-    // 24 bits (bf16 x3 uses 16 significant bits of each), 4 modes per column
-    // interleaved: [region][mode / 4][3 planes][arow] x 8 bytes.  Built
-    // on the first 16-bit exec that needs it (scw::build_a24).
-    scw::DevBuf d_a24;
-    std::vector<int64_t> a24_off;  // byte offset per region, -1 if none
+    // Dense-region amplitudes for the 16-bit GEMM dtypes in block floating
+    // point: per group of 4 modes and column, 8 16-bit mantissas and an
+    // exponent byte per mode, [region]{[mode / 4][arow] x 16 bytes, then
+    // [mode / 4][arow] x 4 bytes}.  Built on the first 16-bit exec that needs
+    // it (scw::build_bfp).
+    scw::DevBuf d_bfp;
+    std::vector<int64_t> bfp_off;  // byte offset per region, -1 if none
 };
 
 namespace scw {
@@ -135,7 +136,8 @@ struct RegionDev {      // per-region data the GEMM-form kernels read (device)
     int64_t x_off;      // float2 offset of X_r ([mx][nx])
     int32_t cells, width, mc, dense;
     int32_t mx, my, nslots, arow;
-    const uint8_t* a24; // 24-bit amplitudes of a dense region (16-bit dtypes), else nullptr
+    const uint8_t* bfp;      // block floating-point mantissas of a dense region (16-bit dtypes), else nullptr
+    const uint8_t* bfp_exp;  // their exponent bytes (4 per record)
 };
 
 // Direct form helpers (scw_direct.hip).
@@ -189,5 +191,5 @@ namespace scw {
 // kernel launchers (scw_direct.hip, scw_gemm.hip)
 int launch_direct(const scw_exec* ex, float* out, hipStream_t s);
 int launch_gemm(const scw_exec* ex, float* out, hipStream_t s, int stage = 0);
-int build_a24(scw_plan* plan);  // fills plan->d_a24 / a24_off once (scw_gemm.hip)
+int build_bfp(scw_plan* plan);  // fills plan->d_bfp / bfp_off once (scw_gemm.hip)
 }  // namespace scw

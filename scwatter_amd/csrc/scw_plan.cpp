@@ -405,7 +405,7 @@ int scw_plan_destroy(scw_plan* plan) {
         plan->d_kf.release();
         plan->d_ellA.release();
         plan->d_ellC.release();
-        plan->d_a24.release();
+        plan->d_bfp.release();
     }
     delete plan;
     return SCW_OK;
@@ -435,7 +435,7 @@ int scw_plan_region_info(const scw_plan* plan, int region, int64_t* n_waves, int
 int scw_plan_device_bytes(const scw_plan* plan, size_t* bytes) {
     if (!plan || !bytes) return fail(SCW_ERR_ARG, "NULL argument");
     *bytes = plan->d_k64.bytes + plan->d_e64.bytes + plan->d_lns.bytes + plan->d_kf.bytes +
-             plan->d_ellA.bytes + plan->d_ellC.bytes + plan->d_a24.bytes;
+             plan->d_ellA.bytes + plan->d_ellC.bytes + plan->d_bfp.bytes;
     return SCW_OK;
 }
 
