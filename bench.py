@@ -9,8 +9,9 @@ Workload (one step = one evaluation of the whole grid, tables resident in HBM):
   15,559,803 waves in all; field on 256 x 256 x 128 points (x, y in [-32, 32],
   z in [-10, 10]); seeded synthetic eigen-tables.
 
-The GEMM form is timed as K replays of a captured hipGraph of one run; the
-direct form (seconds per run) is timed over --direct-steps runs.  Device
+The GEMM form is timed over K back-to-back runs, launched eagerly and as
+replays of a captured hipGraph of one run (the faster counts; both are in the
+line); the direct form (seconds per run) is timed over --direct-steps runs.  Device
 time comes from hipEvents recorded on the stream the kernels run on.  The
 CPU baseline is the reference-semantics FP32 port (single thread, like
 cpuEvaluator.cpp) on a bounded sample of the same workload; the same CPU
@@ -42,23 +43,34 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def time_graph(ex, out, stream, steps, warmup):
+def time_steps(ex, out, stream, steps, warmup):
+    """Device ms per step over `steps` back-to-back runs, launched two ways:
+    eager (scw_exec_run: the stage kernels enqueued directly) and replayed from
+    a captured hipGraph.  Returns (ms of the faster, wall ms of it, which,
+    {way: ms}).  On this runtime a graph replay leaves a few microseconds
+    idle between consecutive replays that eager launches do not."""
     from scwatter_amd import hip
-    ex.graph_capture(out.ptr, stream.handle)
-    for _ in range(warmup):
-        ex.graph_launch(stream.handle)
-    stream.synchronize()
-    hip.synchronize()
     e0, e1 = hip.Event(), hip.Event()
-    t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(steps):
-        ex.graph_launch(stream.handle)
-    e1.record(stream)
-    e1.synchronize()
-    hip.synchronize()
-    wall = time.perf_counter() - t0
-    return e0.elapsed_ms(e1) / steps, wall * 1e3 / steps
+
+    def timed(launch):
+        for _ in range(warmup):
+            launch()
+        stream.synchronize()
+        hip.synchronize()
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(steps):
+            launch()
+        e1.record(stream)
+        e1.synchronize()
+        hip.synchronize()
+        return e0.elapsed_ms(e1) / steps, (time.perf_counter() - t0) * 1e3 / steps
+
+    res = {"eager": timed(lambda: ex.run(out.ptr, stream.handle))}
+    ex.graph_capture(out.ptr, stream.handle)
+    res["graph"] = timed(lambda: ex.graph_launch(stream.handle))
+    way = min(res, key=lambda k: res[k][0])
+    return res[way][0], res[way][1], way, {k: round(v[0], 4) for k, v in res.items()}
 
 
 def time_stages(ex, out, stream, reps):
@@ -153,7 +165,7 @@ def sample_device(out, shape, pidx):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--direct-steps", type=int, default=1)
     ap.add_argument("--sample-points", type=int, default=192)
@@ -210,16 +222,17 @@ def main():
         t0 = time.perf_counter()
         ex = Exec(plan, xs, ys, zs, form="gemm", dtype=dt)
         info = ex.info()
-        ms, wall_ms = time_graph(ex, out, stream, args.steps, args.warmup)
+        ms, wall_ms, way, by_way = time_steps(ex, out, stream, args.steps, args.warmup)
         samp = sample_device(out, shape, pidx)
         stages = {}
         sms = time_stages(ex, out, stream, max(3, args.steps // 2))
         for st, name in ((1, "coef"), (2, "xcontract"), (3, "main")):
             fl, by = ex.stage_info(st)
             stages[name] = {"ms": round(sms[st - 1], 4), "flops": fl, "bytes": by}
-        forms[f"gemm_{dt}"] = {"ms": ms, "wall_ms": wall_ms, "rate": info["units"] / (ms * 1e-3), "info": info,
+        forms[f"gemm_{dt}"] = {"ms": ms, "wall_ms": wall_ms, "launch": way, "ms_by_launch": by_way,
+                               "rate": info["units"] / (ms * 1e-3), "info": info,
                                "stages": stages, "sample": samp}
-        log(f"[bench] gemm {dt}: {ms:.3f} ms/step (wall {wall_ms:.3f}), "
+        log(f"[bench] gemm {dt}: {ms:.3f} ms/step ({way}; {by_way}; wall {wall_ms:.3f}), "
             f"{info['units'] / (ms * 1e-3):.3e} pts*waves/s, stages " +
             ", ".join(f"{k} {v['ms']:.3f} ms" for k, v in stages.items()) + f" ({time.perf_counter()-t0:.1f} s)")
         ex.destroy()
@@ -294,7 +307,9 @@ def main():
         "config": {"workload": desc, "units_per_step": B["info"]["units"], "form": "gemm", "gemm_dtype": dt},
         "rel_l2": B["rel_l2"], "max_abs": B["max_abs"], "accuracy_sample_points": len(pts),
         "gemm_over_direct": B["rate"] / D["rate"],
-        "forms": {k: {"ms_per_step": round(f["ms"], 4), "rate": f["rate"], "rel_l2": f["rel_l2"],
+        "forms": {k: {"ms_per_step": round(f["ms"], 4), **({"launch": f["launch"], "ms_by_launch": f["ms_by_launch"]}
+                                                          if "launch" in f else {}),
+                      "rate": f["rate"], "rel_l2": f["rel_l2"],
                       "max_abs": f["max_abs"], **({"stages_ms": {s: v["ms"] for s, v in f["stages"].items()}}
                                                   if "stages" in f else {})} for k, f in forms.items()},
         "roofline": roof, "roofline_direct": roof_direct,
@@ -349,7 +364,7 @@ def bench_batch(args):
         out = hip.DeviceArray(bx.shape, np.complex64)
         best, cand = autotune.tune(bx, out.ptr, stream, reps=3)
         info = bx.info()
-        ms, wall_ms = time_graph(bx, out, stream, args.steps, args.warmup)
+        ms, wall_ms, way, by_way = time_steps(bx, out, stream, args.steps, args.warmup)
         sms = time_stages(bx, out, stream, 3)
         stages = {}
         for st, name in ((1, "coef"), (2, "xcontract"), (3, "main")):
@@ -357,7 +372,8 @@ def bench_batch(args):
             stages[name] = {"ms": round(sms[st - 1], 4), "flops": fl, "bytes": by}
         pidx = np.array([(i * nzi + z, y, x) for i, z, y, x in samples])
         samp = sample_device(out, bx.shape, pidx)
-        forms[f"gemm_{dt}"] = {"ms": ms, "wall_ms": wall_ms, "rate": info["units"] / (ms * 1e-3), "info": info,
+        forms[f"gemm_{dt}"] = {"ms": ms, "wall_ms": wall_ms, "launch": way, "ms_by_launch": by_way,
+                               "rate": info["units"] / (ms * 1e-3), "info": info,
                                "stages": stages, "sample": samp, "tuned": {"coef_items": best[0], "main_blocks": best[1],
                                                                            "candidates_ms": {f"{a}/{b}": round(v, 4)
                                                                                              for (a, b), v in cand.items()}}}
@@ -395,7 +411,9 @@ def bench_batch(args):
         "config": {"workload": WORKLOADS["c4"][0], "units_per_step": B["info"]["units"], "form": "gemm",
                    "gemm_dtype": dt, "instances": len(plans), "waves": waves},
         "rel_l2": B["rel_l2"], "max_abs": B["max_abs"], "accuracy_sample_points": len(samples),
-        "forms": {k: {"ms_per_step": round(f["ms"], 4), "rate": f["rate"], "rel_l2": f["rel_l2"],
+        "forms": {k: {"ms_per_step": round(f["ms"], 4), **({"launch": f["launch"], "ms_by_launch": f["ms_by_launch"]}
+                                                          if "launch" in f else {}),
+                      "rate": f["rate"], "rel_l2": f["rel_l2"],
                       "max_abs": f["max_abs"], "stages_ms": {s: v["ms"] for s, v in f["stages"].items()},
                       "tuned": f["tuned"]} for k, f in forms.items()},
         "roofline": roof,

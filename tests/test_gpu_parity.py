@@ -254,6 +254,31 @@ def test_graph_replay_matches_eager(c1):
         s.destroy()
 
 
+@pytest.mark.parametrize("dtype", ["bf16x3", "bf16"])
+def test_graph_replay_with_both_coefficient_kernels(tile_small, dtype):
+    """The tile-sample grid crosses the half-spaces (VALU term contraction) and
+    the sublayers (MFMA mode contraction): eager runs and replays of the
+    captured graph, with other work on the device between them, give the same
+    field, which matches the oracle."""
+    from scwatter_amd import hip
+    plan, g, ref = tile_small
+    ex = Exec(plan, *g, form="gemm", dtype=dtype)
+    eager = ex.run_host()
+    _check(eager, ref, TOL[dtype], f"tile sample graph {dtype}")
+    s = hip.Stream()
+    buf = hip.DeviceArray(ex.shape, np.complex64)
+    ex.graph_capture(buf.ptr, s.handle)
+    for _ in range(3):
+        hip.DeviceArray.from_host(np.zeros(4, np.float32))  # eager work between replays
+        ex.graph_launch(s.handle)
+        s.synchronize()
+        assert np.array_equal(buf.to_host(), eager)
+        ex.run(buf.ptr, s.handle)  # eager run on the caller's stream
+        s.synchronize()
+        assert np.array_equal(buf.to_host(), eager)
+    s.destroy()
+
+
 def test_errors_are_reported():
     from scwatter_amd import ScwError
     from scwatter_amd.cw import WaveTables
