@@ -141,6 +141,7 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
     std::vector<uint16_t> yhi, ylo;
     int kmax = 0;
     ex->my_max = 0;
+    ex->ka_max = 0;
     for (int v = 0; v < NV; ++v) {
         const scw_plan* P = vreg[v].first;
         const int r = vreg[v].second;
@@ -166,15 +167,21 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
         d.my = R.my;
         d.nslots = 1;
         ex->my_max = std::max(ex->my_max, R.my);
-        // X_r[a][x] = exp(i kx_a x)
+        if (R.my > 0) ex->ka_max = std::max(ex->ka_max, (R.mx + 15) / 16 * 16);
+        // X_r[a][x] = exp(i kx_a x): [mx][nx] for the FP32 X-contraction
+        // kernel; x-major [nx][ka] (zero past mx) for the 16-bit dtypes, whose
+        // main GEMM reads four consecutive a of one x per lane
+        d.ka = (R.mx + 15) / 16 * 16;
+        d.pad_ = 0;
         d.x_off = (int64_t)X.size() / 2;
-        X.resize(X.size() + (size_t)2 * R.mx * nx);
+        X.resize(X.size() + (size_t)2 * (is16 ? (int64_t)d.ka : R.mx) * nx, 0.0f);
         for (int a = 0; a < R.mx; ++a)
             for (int i = 0; i < nx; ++i) {
                 const cd val = std::exp(cd(0, 1) * R.kx[a] * xs[i]);
                 if (!region_ok(val)) return fail(SCW_ERR_UNSUPPORTED, "region %d: exp(i kx x) outside FP32 range", r);
-                X[2 * (d.x_off + (int64_t)a * nx + i)] = (float)val.real();
-                X[2 * (d.x_off + (int64_t)a * nx + i) + 1] = (float)val.imag();
+                const int64_t e = d.x_off + (is16 ? (int64_t)i * d.ka + a : (int64_t)a * nx + i);
+                X[2 * e] = (float)val.real();
+                X[2 * e + 1] = (float)val.imag();
             }
         // Y fragments: Yp[y][2b] = Re exp(i ky_b y), Yp[y][2b+1] = Im.
         const int kp = R.my > 0 ? round_up(2 * R.my, kq) : 0;
@@ -222,9 +229,9 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
         ex->kmax = kmax <= 16 ? 16 : kmax <= 32 ? 32 : kmax <= 64 ? 64 : 128;
     else
         ex->kmax = kmax <= 32 ? 32 : kmax <= 64 ? 64 : kmax <= 96 ? 96 : 128;
-    // The MFMA GEMM reads one z-slice of U (up to kmax / 2 rows of 6 nx
+    // The FP32 MFMA GEMM reads one z-slice of U (up to kmax / 2 rows of 6 nx
     // floats, plus a lane overhang past the last column) with 32-bit offsets.
-    if (((double)ex->kmax / 2 * 6.0 * nx + 512.0) * 4.0 >= 2147483648.0)
+    if (!is16 && ((double)ex->kmax / 2 * 6.0 * nx + 512.0) * 4.0 >= 2147483648.0)
         return fail(SCW_ERR_UNSUPPORTED, "GEMM form: nx = %d is too large for %d distinct ky (U slice over 2 GiB)",
                     nx, kmax / 2);
 
@@ -299,7 +306,7 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
         m.yfrag_off = yfrag_off[v];
         m.zrow_off = (int64_t)Z.size() / 2;
         m.c_off = c_off;
-        u_off += (int64_t)R.my * nx * 3;
+        u_off += is16 ? 3LL * R.my * rd[v].ka : (int64_t)R.my * nx * 3;
         c_off += (int64_t)R.mx * R.my * 3;
         for (int c = 0; c < R.mc; ++c) {
             const cd val = std::exp(cd(0, 1) * R.kz[c] * (ze[z].z - R.zref));
@@ -310,13 +317,22 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
             Z.push_back((float)val.imag());
         }
         f_coef += 24.0 * (double)R.n_terms;
-        f_u += 24.0 * (double)R.mx * R.my * nx;
         f_main += 24.0 * (double)R.my * nx * ny;
         const double cbytes = 24.0 * (double)R.mx * R.my;  // C_z, one slot
-        const double ubytes = 24.0 * (double)R.my * nx;     // U_z
         b_coef += cbytes * rd[v].nslots;
-        b_u += cbytes * rd[v].nslots + ubytes + 8.0 * R.mx * nx;
-        b_main += ubytes;
+        if (!is16) {
+            // X contraction kernel: slots and X in, U_z out; the GEMM reads U_z
+            const double ubytes = 24.0 * (double)R.my * nx;
+            f_u += 24.0 * (double)R.mx * R.my * nx;
+            b_u += cbytes * rd[v].nslots + ubytes + 8.0 * R.mx * nx;
+            b_main += ubytes;
+        } else {
+            // slot sum: slots in, C_z out; the GEMM reads C_z and X and runs
+            // the X contraction of its columns itself
+            f_main += 24.0 * (double)R.mx * R.my * nx;
+            b_u += cbytes * rd[v].nslots + cbytes;
+            b_main += cbytes + 8.0 * R.mx * nx;
+        }
     }
     ex->c_elems = c_off;
 

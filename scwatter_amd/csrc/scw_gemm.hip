@@ -544,8 +544,192 @@ __device__ inline floatx4 mfma16(const typename Op16<MODE>::vec a, const typenam
         return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
+// ---------------------------------------------------------------------------
+// X contraction inside the 16-bit main GEMM.  A slot-sum kernel first adds
+// the coefficient partials of every z in slot order into
+//   Cs_z[m][a] = C_z[a, b = m / 3, comp = m % 3],  m < 3 my, a < ka (zero past mx),
+// and each main-GEMM workgroup then computes U_z[b][p] = sum_a Cs_z[3 b + comp][a]
+// X[a][x] for its own 128 column pairs p = 3 x + comp on MFMA, into LDS, so
+// U_z never goes through HBM.  The GEMM is real, with rows (m, re|im), K =
+// (a, re|im) and columns x; the A operand holds the signed parts of C (re,re =
+// Re C, re,im = -Im C, im,re = Im C, im,im = Re C), the B operand X, both split
+// into bf16 hi + lo (bf16 x3: hi hi + hi lo + lo hi), as the FP32 X
+// contraction kernel u_mfma<1> did.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void csum_kernel(const ZMeta* __restrict__ zmeta,
+                                                   const RegionDev* __restrict__ regions,
+                                                   const float2* __restrict__ C, int64_t c_elems,
+                                                   float2* __restrict__ Cs, int zoff) {
+    const ZMeta zm = zmeta[blockIdx.y + zoff];
+    const int ka = regions[zm.region].ka, ns = regions[zm.region].nslots;
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= 3 * zm.my * ka) return;
+    const int m = e / ka, a = e - m * ka;
+    float2 sum = make_float2(0.0f, 0.0f);
+    if (a < zm.mx) {
+        const int64_t off = zm.c_off + ((int64_t)(m / 3) * zm.mx + a) * 3 + (m % 3);
+        for (int k0 = 0; k0 < ns; k0 += 16) {
+            float2 v[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u)
+                v[u] = k0 + u < ns ? C[(int64_t)(k0 + u) * c_elems + off] : make_float2(0.0f, 0.0f);
+#pragma unroll
+            for (int u = 0; u < 16; ++u)
+                if (k0 + u < ns) {
+                    sum.x += v[u].x;
+                    sum.y += v[u].y;
+                }
+        }
+    }
+    Cs[zm.u_off + e] = sum;
+}
+
+constexpr int UPL = 130;  // LDS row pitch (float2) of a workgroup's U_z columns
+// K steps (16 a each) whose X fragments a workgroup stages in LDS: 3 (mx <=
+// 48) up to KMAX = 96; none for KMAX = 128, whose U_z columns alone take
+// 66.5 KB (two workgroups per CU).
+template <int KMAX>
+constexpr int xt_staged() { return KMAX <= 96 ? 3 : 0; }
+typedef uint32x4 XFrag[2];    // B-operand fragment of one lane: bf16 x 8 high, then low
+
+__device__ inline void split_bf16(float v, __bf16& h, __bf16& l) {
+    h = (__bf16)v;
+    l = (__bf16)(v - (float)h);
+}
+
+// A operand of the X contraction for row m and K step t: the signed parts of
+// C_z[m][16 t + 4 g + h] (c01 = a, a + 1; c23 = a + 2, a + 3; re, im each),
+// row part o = 0 -> (Re, -Im), o = 1 -> (Im, Re), split into bf16 hi / lo.
+__device__ inline void xc_a_operand(const floatx4 c01, const floatx4 c23, int o, bf16x8& ah, bf16x8& al) {
+    const float cre[4] = {c01[0], c01[2], c23[0], c23[2]}, cim[4] = {c01[1], c01[3], c23[1], c23[3]};
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int h = e >> 1;
+        const float v = o == 0 ? ((e & 1) ? -cim[h] : cre[h]) : ((e & 1) ? cre[h] : cim[h]);
+        __bf16 hh, ll;
+        split_bf16(v, hh, ll);
+        ah[e] = hh;
+        al[e] = ll;
+    }
+}
+// B operand: X[a][x] for a = 16 t + 4 g + e / 2, part e % 2 (q0 = a, a + 1; q1 = a + 2, a + 3).
+__device__ inline void xc_b_operand(const floatx4 q0, const floatx4 q1, bf16x8& bh, bf16x8& bl) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const float v = e < 4 ? q0[e] : q1[e - 4];
+        __bf16 hh, ll;
+        split_bf16(v, hh, ll);
+        bh[e] = hh;
+        bl[e] = ll;
+    }
+}
+__device__ inline void xc_mfma(floatx4& acc, const bf16x8& ah, const bf16x8& al, const bf16x8& bh, const bf16x8& bl) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
+}
+
+// U_z[b][128 blockIdx.x + pl] for pl < 128 and b < my into Ul[b][pl].  Wave
+// wv takes row tiles wv, wv + 4, ... (8 rows m each) against the 3 column
+// tiles of 16 x from x_lo = 128 blockIdx.x / 3 on (pl = 3 (x - x_lo) + comp -
+// r0 covers 0..127 with x - x_lo < 44).  Rows past 3 my and x past nx
+// re-read the last row / column (results not stored / only for columns the
+// stores drop).  With at most XS K steps (mx <= 16 XS) the workgroup
+// first stages the X fragments, already split, in LDS (xs), and each row tile
+// issues all its C loads at once: one L2 round trip per row tile instead of
+// one per K step; larger mx takes the K steps one at a time.
+template <int XS>
+__device__ inline void xcontract_columns(const ZMeta& zm, const RegionDev& R, const float2* __restrict__ Cs,
+                                         const float2* __restrict__ X, int nx, float2 (*Ul)[UPL], XFrag* xs) {
+    const int lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4, o = j & 1;
+    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int p0 = 128 * (int)blockIdx.x, x_lo = p0 / 3, r0 = p0 - 3 * x_lo;
+    const int M = 3 * zm.my, RT = (M + 7) >> 3, T = R.ka >> 4;
+    const float2* Cz = Cs + zm.u_off;
+    const float2* Xr = X + R.x_off;
+    auto xload = [&](int ct, int t, floatx4& q0, floatx4& q1) {
+        const int x = min(x_lo + 16 * ct + j, nx - 1);
+        const floatx4* xp = reinterpret_cast<const floatx4*>(Xr + (int64_t)x * R.ka + 16 * t + 4 * g);
+        q0 = xp[0];
+        q1 = xp[1];
+    };
+    auto store = [&](const floatx4 (&acc)[3], int rt) {
+        // lane holds D[i = 4 g + r][x - x_lo = 16 ct + j]: rows m = 8 rt + 2 g + r / 2, part r % 2
+#pragma unroll
+        for (int ct = 0; ct < 3; ++ct)
+#pragma unroll
+            for (int hh = 0; hh < 2; ++hh) {
+                const int mm = 8 * rt + 2 * g + hh;
+                const int b = mm / 3, comp = mm - 3 * b;
+                const int pl = 3 * (16 * ct + j) + comp - r0;
+                if (mm < M && pl >= 0 && pl < 128) Ul[b][pl] = make_float2(acc[ct][2 * hh], acc[ct][2 * hh + 1]);
+            }
+    };
+    if (XS > 0 && T <= XS) {  // uniform
+        for (int f = threadIdx.x; f < 3 * T * 64; f += 256) {
+            const int l = f & 63, ct = (f >> 6) % 3, t = (f >> 6) / 3;
+            const int x = min(x_lo + 16 * ct + (l & 15), nx - 1);
+            const floatx4* xp = reinterpret_cast<const floatx4*>(Xr + (int64_t)x * R.ka + 16 * t + 4 * (l >> 4));
+            bf16x8 bh, bl;
+            xc_b_operand(xp[0], xp[1], bh, bl);
+            xs[f][0] = __builtin_bit_cast(uint32x4, bh);
+            xs[f][1] = __builtin_bit_cast(uint32x4, bl);
+        }
+        __syncthreads();
+        for (int rt = wv; rt < RT; rt += 4) {
+            const int m = min(8 * rt + (j >> 1), M - 1);
+            floatx4 cv[XS > 0 ? XS : 1][2];
+#pragma unroll
+            for (int t = 0; t < XS; ++t) {  // unconditional: steps past T re-read the last one
+                const floatx4* cp =
+                    reinterpret_cast<const floatx4*>(Cz + (int64_t)m * R.ka + 16 * min(t, T - 1) + 4 * g);
+                cv[t][0] = cp[0];
+                cv[t][1] = cp[1];
+            }
+            floatx4 acc[3];
+#pragma unroll
+            for (int ct = 0; ct < 3; ++ct) acc[ct] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int t = 0; t < XS; ++t) {
+                if (t >= T) break;  // uniform
+                bf16x8 ah, al;
+                xc_a_operand(cv[t][0], cv[t][1], o, ah, al);
+#pragma unroll
+                for (int ct = 0; ct < 3; ++ct) {
+                    const XFrag& xf = xs[(t * 3 + ct) * 64 + lane];
+                    xc_mfma(acc[ct], ah, al, __builtin_bit_cast(bf16x8, xf[0]), __builtin_bit_cast(bf16x8, xf[1]));
+                }
+            }
+            store(acc, rt);
+        }
+    } else {
+        for (int rt = wv; rt < RT; rt += 4) {
+            const int m = min(8 * rt + (j >> 1), M - 1);
+            floatx4 acc[3];
+#pragma unroll
+            for (int ct = 0; ct < 3; ++ct) acc[ct] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+            for (int t = 0; t < T; ++t) {
+                const floatx4* cp = reinterpret_cast<const floatx4*>(Cz + (int64_t)m * R.ka + 16 * t + 4 * g);
+                floatx4 xv[3][2];
+#pragma unroll
+                for (int ct = 0; ct < 3; ++ct) xload(ct, t, xv[ct][0], xv[ct][1]);
+                bf16x8 ah, al;
+                xc_a_operand(cp[0], cp[1], o, ah, al);
+#pragma unroll
+                for (int ct = 0; ct < 3; ++ct) {
+                    bf16x8 bh, bl;
+                    xc_b_operand(xv[ct][0], xv[ct][1], bh, bl);
+                    xc_mfma(acc[ct], ah, al, bh, bl);
+                }
+            }
+            store(acc, rt);
+        }
+    }
+}
+
 template <int KMAX, bool VEC, int MODE>
-__global__ __launch_bounds__(256) void main_16(const ZMeta* __restrict__ zmeta, const float2* __restrict__ U,
+__global__ __launch_bounds__(256) void main_16(const ZMeta* __restrict__ zmeta, const RegionDev* __restrict__ regions,
+                                               const float2* __restrict__ Cs, const float2* __restrict__ X,
                                                const uint16_t* __restrict__ Yfrag, int64_t ylo_delta, int nx,
                                                int ny, int ytiles_per_split, float* __restrict__ out, int zoff) {
     typedef typename Op16<MODE>::vec vec;
@@ -558,7 +742,6 @@ __global__ __launch_bounds__(256) void main_16(const ZMeta* __restrict__ zmeta, 
     const int64_t ncols = 6LL * nx;
     const int64_t n0 = (int64_t)blockIdx.x * 256 + wv * 64;  // waves past ncols compute zeros
     const int li = lane & 15, lk = lane >> 4;
-    const __amdgpu_buffer_rsrc_t urs = u_slice_rsrc(reinterpret_cast<const float*>(U + zm.u_off), zm.my, ncols);
     const int ytiles = (ny + 15) >> 4;
     const int yt0 = blockIdx.z * ytiles_per_split;
     const int yt1 = min(ytiles, yt0 + ytiles_per_split);
@@ -567,24 +750,40 @@ __global__ __launch_bounds__(256) void main_16(const ZMeta* __restrict__ zmeta, 
         zero_tiles<VEC>(oz, ncols, ny, yt0, yt1, (int64_t)blockIdx.x * 256);
         return;
     }
-    __shared__ float T[2][16][TROW];
+    // LDS: the workgroup's U_z columns while the fragments are built, then the
+    // double-buffered output transpose
+    constexpr int kTBytes = 2 * 16 * TROW * 4, kUBytes = KMAX / 2 * UPL * 8;
+    constexpr int kXS = xt_staged<KMAX>(), kXBytes = 3 * kXS * 64 * (int)sizeof(XFrag);
+    constexpr int kBytes = kTBytes > kUBytes + kXBytes ? kTBytes : kUBytes + kXBytes;
+    __shared__ __attribute__((aligned(16))) unsigned char lds[kBytes];
+    float (*T)[16][TROW] = reinterpret_cast<float (*)[16][TROW]>(lds);
+    float2 (*Ul)[UPL] = reinterpret_cast<float2 (*)[UPL]>(lds);
+    XFrag* xs = reinterpret_cast<XFrag*>(lds + kUBytes);
+    xcontract_columns<kXS>(zm, regions[zm.region], Cs, X, nx, Ul, xs);
+    __syncthreads();
 
+    // A fragment: Bp[k][n] from U_z[b = k >> 1][n >> 1]; rows past my are zero
     vec ahi[4][S], alo[4][S];
 #pragma unroll
     for (int it = 0; it < 4; ++it) {
         const int64_t n = n0 + 16 * it + li;
+        const int pl = (64 * wv + 16 * it + li) >> 1;
+        const bool odd = (n & 1) != 0;
 #pragma unroll
         for (int s = 0; s < S; ++s) {
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                const int k = 32 * s + 8 * lk + e;
-                const float v = bp_buffer(urs, ncols, k, n);
+                const int k = 32 * s + 8 * lk + e, b = k >> 1;
+                const float2 u = Ul[min(b, KMAX / 2 - 1)][pl];
+                const float bp = (k & 1) == 0 ? (odd ? u.y : u.x) : (odd ? u.x : -u.y);
+                const float v = b < zm.my ? bp : 0.0f;
                 const elem h = (elem)v;
                 ahi[it][s][e] = h;
                 if constexpr (MODE == 1) alo[it][s][e] = (elem)(v - (float)h);
             }
         }
     }
+    __syncthreads();  // the transpose buffers reuse the U_z columns
 
     const uint16_t* Yf = Yfrag + zm.yfrag_off + (int64_t)lane * 8;
     // Unconditional Y loads and a loop unrolled by two (see main_f32).
@@ -644,15 +843,17 @@ static void launch_main(const scw_exec* ex, dim3 grid, hipStream_t st, int ytps,
     }
     if constexpr (KMAX >= 32) {
         const uint16_t* Y = ex->d_Yfrag.as<uint16_t>();
+        const RegionDev* rg = ex->d_regions.as<RegionDev>();
+        const float2* X = ex->d_X.as<float2>();
         if (ex->dtype == SCW_DTYPE_BF16)
-            hipLaunchKernelGGL((main_16<KMAX, VEC, 0>), grid, dim3(256), 0, st, zm, U, Y, ex->ylo_delta, nx, ny,
-                               ytps, out, zoff);
+            hipLaunchKernelGGL((main_16<KMAX, VEC, 0>), grid, dim3(256), 0, st, zm, rg, U, X, Y, ex->ylo_delta, nx,
+                               ny, ytps, out, zoff);
         else if (ex->dtype == SCW_DTYPE_BF16X3)
-            hipLaunchKernelGGL((main_16<KMAX, VEC, 1>), grid, dim3(256), 0, st, zm, U, Y, ex->ylo_delta, nx, ny,
-                               ytps, out, zoff);
+            hipLaunchKernelGGL((main_16<KMAX, VEC, 1>), grid, dim3(256), 0, st, zm, rg, U, X, Y, ex->ylo_delta, nx,
+                               ny, ytps, out, zoff);
         else
-            hipLaunchKernelGGL((main_16<KMAX, VEC, 2>), grid, dim3(256), 0, st, zm, U, Y, ex->ylo_delta, nx, ny,
-                               ytps, out, zoff);
+            hipLaunchKernelGGL((main_16<KMAX, VEC, 2>), grid, dim3(256), 0, st, zm, rg, U, X, Y, ex->ylo_delta, nx,
+                               ny, ytps, out, zoff);
     }
 }
 
@@ -1103,9 +1304,9 @@ int launch_gemm(const scw_exec* ex, float* out, hipStream_t st, int stage) {
                                ex->d_C.as<float2>(), ex->c_elems, ex->d_X.as<float2>(), (int)ex->nx,
                                ex->d_U.as<float2>(), 0);
         else
-            hipLaunchKernelGGL(u_mfma<1>, g, dim3(256), 0, st, ex->d_zmeta.as<ZMeta>(), ex->d_regions.as<RegionDev>(),
-                               ex->d_C.as<float2>(), ex->c_elems, ex->d_X.as<float2>(), (int)ex->nx,
-                               ex->d_U.as<float2>(), 0);
+            hipLaunchKernelGGL(csum_kernel, dim3((unsigned)((3 * ex->my_max * ex->ka_max + 255) / 256), (unsigned)ex->nz),
+                               dim3(256), 0, st, ex->d_zmeta.as<ZMeta>(), ex->d_regions.as<RegionDev>(),
+                               ex->d_C.as<float2>(), ex->c_elems, ex->d_U.as<float2>(), 0);
         SCW_HIP(hipGetLastError());
     }
     if (stage != 0 && stage != 3) return SCW_OK;

@@ -101,7 +101,8 @@ struct ZMeta {
     int32_t my;         // distinct ky of the region
     int32_t mx;         // distinct kx of the region
     int32_t kp;         // padded K of the main GEMM (2*my rounded up to the MFMA K)
-    int64_t u_off;      // float2 offset of U_z ([my][nx][3])
+    int64_t u_off;      // float2 offset of U_z ([my][nx][3], FP32), or of the slot-summed
+                        // C_z ([3 my][ka], 16-bit dtypes, zero past mx)
     int64_t yfrag_off;  // element offset of the region's Y fragment table
     int64_t zrow_off;   // float2 offset of this z's row of exp(i kz_c (z - zref))
     int64_t c_off;      // float2 offset of C_z ([cells][3]) within one partial-sum slot
@@ -133,9 +134,10 @@ struct ModalItem {       // one wave of the MFMA mode contraction
 struct RegionDev {      // per-region data the GEMM-form kernels read (device)
     const float2* ellA; // the region's terms: [width][arow], row j = [cells][3] float2
     const int32_t* ellC;// kz index per term slot (nullptr when dense)
-    int64_t x_off;      // float2 offset of X_r ([mx][nx])
+    int64_t x_off;      // float2 offset of X_r: [mx][nx] (FP32) or [nx][ka] (16-bit dtypes, zero past mx)
     int32_t cells, width, mc, dense;
     int32_t mx, my, nslots, arow;
+    int32_t ka, pad_;   // mx rounded up to 16 (the X contraction's K in complex steps)
     const uint8_t* bfp;      // block floating-point mantissas of a dense region (16-bit dtypes), else nullptr
     const uint8_t* bfp_exp;  // their exponent bytes (4 per record)
 };
@@ -165,6 +167,7 @@ struct scw_exec {
     // GEMM form
     int kmax = 0;             // largest padded K
     int my_max = 0;
+    int ka_max = 0;           // largest mx rounded up to 16 (regions with waves)
     int nslots_max = 1;
     std::vector<scw::CoefItem> items;
     std::vector<scw::ModalItem> mitems;   // grouped by row-tile class (MT = 1, 2, 4, 8)

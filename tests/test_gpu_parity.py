@@ -153,6 +153,22 @@ def test_dense_slab_mode_contraction(slab_dense, dtype, monkeypatch):
     _check(Exec(plan, *g, form="gemm", dtype=dtype).run_host(), ref, TOL[dtype], f"dense slab VALU {dtype}")
 
 
+@pytest.mark.parametrize("dtype", ["bf16x3", "bf16", "fp32"])
+def test_wide_pupil(dtype):
+    """51 x 51 orders: more than 48 kx and ky per region, so the 16-bit main
+    GEMM runs with K = 128 and its in-kernel X contraction takes the K steps
+    one at a time (no staged X fragments); FP32 runs the separate X
+    contraction kernel."""
+    cw = configs.layered_structure([1.0, 1.4 + 0.05j], [0.0], 1.0, 0.5, 51, ("x",))
+    t = cw.unpack()
+    plan = Plan(t)
+    info = plan.region_info(0)
+    assert info["separable"] and info["mx"] > 48 and info["my"] > 48
+    xs, ys, zs = configs.axis(-20, 20, 37), configs.axis(-18, 18, 21), configs.axis(-5, 5, 6)
+    ref = oracle.eval_grid_f64(t.W, t.z_layers, t.begin, t.end, xs, ys, zs)
+    _check(Exec(plan, xs, ys, zs, form="gemm", dtype=dtype).run_host(), ref, TOL[dtype], f"wide pupil {dtype}")
+
+
 @pytest.fixture(scope="module")
 def stack_small():
     rng = np.random.default_rng(11)
