@@ -122,7 +122,7 @@ def stage_kernels(stage, dt):
     coef = [r"scw::coef_kernel", r"scw::modal_kernel<%d, \d>" % mode]
     if dt != "fp32":
         coef.append(r"scw::modal_wg_kernel<%d>" % mode)
-    xc = r"scw::u_mfma<%d>" % (3 if dt == "fp32" else 1)
+    xc = r"scw::u_mfma" if dt == "fp32" else r"scw::csum_kernel"  # 16-bit: slot sum; X inside main_16
     return {"coef": coef, "xcontract": [xc], "main": [main]}[stage]
 
 

@@ -9,10 +9,14 @@
 //     C_z[a,b]       = sum_c A[a,b,c] exp(i kz_c (z - zref)),
 //     X[i,a] = exp(i kx_a x_i),  Y[j,b] = exp(i ky_b y_j).
 //
-// Three launches per grid:
-//   coef_kernel : C_z for every z of the grid (ELL contraction over the
-//                 region's terms; wave-level work items, FP32 VALU).
-//   u_mfma      : U_z[b][x][comp] = sum_a C_z[a,b,comp] X[a][x]  (MFMA).
+// Three stages per grid:
+//   coefficients: C_z for every z of the grid, in partial-sum slots
+//                 (coef_kernel: VALU ELL contraction of the region's terms;
+//                 modal_kernel / modal_wg_kernel: MFMA mode contraction of
+//                 dense eigenmode regions).
+//   X contraction: U_z[b][x][comp] = sum_a C_z[a,b,comp] X[a][x]; FP32:
+//                 u_mfma (MFMA), U_z through HBM; 16-bit dtypes: csum_kernel
+//                 adds the slots, and main_16 contracts X for its own columns.
 //   main_*      : E_z[y][x][comp] = sum_b Y[y,b] U_z[b][x][comp] on MFMA.
 //
 // The main GEMM is a real GEMM with the complex product folded into K:
@@ -207,20 +211,19 @@ __global__ __launch_bounds__(256) void coef_kernel(const CoefItem* __restrict__ 
 constexpr int MX_MAX = 128;
 
 // ---------------------------------------------------------------------------
-// X contraction on MFMA: per z, U_z[b][x][comp] = sum_a C_z[a,b,comp] X[a][x]
-// is a real GEMM with rows i = (m = 3 b + comp, re|im), K = (a, re|im) and
-// columns x.  The A operand holds the signed parts of C (re,re = Re C,
-// re,im = -Im C, im,re = Im C, im,im = Re C), taken from LDS after the
-// partial-sum slots are added in a fixed order; the B operand is X as stored.
-// K mapping per 16 a as in the mode contraction (MODE 1: lane k = 8 g + e <->
-// a = 4 g + e / 2; MODE 3: MFMA q, lane k = g <-> a = g + 4 (q / 2)).  MODE 1
-// (bf16 x3) serves the 16-bit dtypes, MODE 3 (FP32) the FP32 dtype.
+// X contraction of the FP32 dtype on MFMA (v_mfma_f32_16x16x4_f32): per z,
+// U_z[b][x][comp] = sum_a C_z[a,b,comp] X[a][x] is a real GEMM with rows
+// i = (m = 3 b + comp, re|im), K = (a, re|im) and columns x.  The A operand
+// holds the signed parts of C (re,re = Re C, re,im = -Im C, im,re = Im C,
+// im,im = Re C), taken from LDS after the partial-sum slots are added in a
+// fixed order; the B operand is X as stored.  K mapping per 16 a as in the
+// mode contraction: MFMA q, lane k = g <-> a = g + 4 (q / 2).  (The 16-bit
+// dtypes run their X contraction inside the main GEMM.)
 // Workgroup = 8 b (24 rows m) x 256 x of one z; wave = 3 row tiles x 64 x.
 // ---------------------------------------------------------------------------
 constexpr int UB = 8;              // b per workgroup
 constexpr int UPITCH = MX_MAX + 2; // LDS row pitch in float2 (rows stay 16-byte aligned)
 
-template <int MODE>
 __global__ __launch_bounds__(256) void u_mfma(const ZMeta* __restrict__ zmeta, const RegionDev* __restrict__ regions,
                                               const float2* __restrict__ C, int64_t c_elems,
                                               const float2* __restrict__ X, int nx, float2* __restrict__ U, int zoff) {
@@ -272,7 +275,7 @@ __global__ __launch_bounds__(256) void u_mfma(const ZMeta* __restrict__ zmeta, c
         float2 xa[4][4], cv[3][4];
 #pragma unroll
         for (int h = 0; h < 4; ++h) {
-            const int a = 16 * t + (MODE == 3 ? g + 4 * h : 4 * g + h);
+            const int a = 16 * t + g + 4 * h;
             const float2* xrow = Xr + (int64_t)min(a, mx - 1) * nx;
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt) xa[nt][h] = xrow[xc[nt]];
@@ -281,46 +284,16 @@ __global__ __launch_bounds__(256) void u_mfma(const ZMeta* __restrict__ zmeta, c
         }
         // signed part of C for (h, s): o = 0 -> (Re, -Im), o = 1 -> (Im, Re)
         auto cpart = [&](const float2 c, int s) { return o == 0 ? (s == 0 ? c.x : -c.y) : (s == 0 ? c.y : c.x); };
-        if constexpr (MODE == 3) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q)
-#pragma unroll
-                for (int nt = 0; nt < 4; ++nt) {
-                    const float bv = (q & 1) ? xa[nt][q >> 1].y : xa[nt][q >> 1].x;
-#pragma unroll
-                    for (int mt = 0; mt < 3; ++mt)
-                        acc[nt][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(cpart(cv[mt][q >> 1], q & 1), bv,
-                                                                            acc[nt][mt], 0, 0, 0);
-                }
-        } else {
-            bf16x8 ah[3], al[3];
-#pragma unroll
-            for (int mt = 0; mt < 3; ++mt)
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const float v = cpart(cv[mt][e >> 1], e & 1);
-                    const __bf16 hv = (__bf16)v;
-                    ah[mt][e] = hv;
-                    al[mt][e] = (__bf16)(v - (float)hv);
-                }
+        for (int q = 0; q < 8; ++q)
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt) {
-                bf16x8 bh, bl;
+                const float bv = (q & 1) ? xa[nt][q >> 1].y : xa[nt][q >> 1].x;
 #pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const float v = (e & 1) ? xa[nt][e >> 1].y : xa[nt][e >> 1].x;
-                    const __bf16 hv = (__bf16)v;
-                    bh[e] = hv;
-                    bl[e] = (__bf16)(v - (float)hv);
-                }
-#pragma unroll
-                for (int mt = 0; mt < 3; ++mt) {
-                    acc[nt][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mt], bl, acc[nt][mt], 0, 0, 0);
-                    acc[nt][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[mt], bh, acc[nt][mt], 0, 0, 0);
-                    acc[nt][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mt], bh, acc[nt][mt], 0, 0, 0);
-                }
+                for (int mt = 0; mt < 3; ++mt)
+                    acc[nt][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(cpart(cv[mt][q >> 1], q & 1), bv,
+                                                                        acc[nt][mt], 0, 0, 0);
             }
-        }
     }
     // lane holds D[i = 4 g + r][x]: rows (m = 8 mt + 2 g + r / 2, part r % 2)
 #pragma unroll
@@ -554,7 +527,7 @@ __device__ inline floatx4 mfma16(const typename Op16<MODE>::vec a, const typenam
 // (a, re|im) and columns x; the A operand holds the signed parts of C (re,re =
 // Re C, re,im = -Im C, im,re = Im C, im,im = Re C), the B operand X, both split
 // into bf16 hi + lo (bf16 x3: hi hi + hi lo + lo hi), as the FP32 X
-// contraction kernel u_mfma<1> did.
+// contraction kernel did before it moved here.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void csum_kernel(const ZMeta* __restrict__ zmeta,
                                                    const RegionDev* __restrict__ regions,
@@ -1300,7 +1273,7 @@ int launch_gemm(const scw_exec* ex, float* out, hipStream_t st, int stage) {
     if (ex->my_max > 0 && (stage == 0 || stage == 2)) {
         const dim3 g((unsigned)((ex->nx + 255) / 256), (unsigned)((ex->my_max + UB - 1) / UB), (unsigned)ex->nz);
         if (ex->dtype == SCW_DTYPE_FP32)
-            hipLaunchKernelGGL(u_mfma<3>, g, dim3(256), 0, st, ex->d_zmeta.as<ZMeta>(), ex->d_regions.as<RegionDev>(),
+            hipLaunchKernelGGL(u_mfma, g, dim3(256), 0, st, ex->d_zmeta.as<ZMeta>(), ex->d_regions.as<RegionDev>(),
                                ex->d_C.as<float2>(), ex->c_elems, ex->d_X.as<float2>(), (int)ex->nx,
                                ex->d_U.as<float2>(), 0);
         else
