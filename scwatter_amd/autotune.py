@@ -11,7 +11,7 @@ import os
 
 from . import hip
 
-CANDIDATES = ((4096, 1536), (2048, 1536), (8192, 1536), (16384, 1536), (4096, 512), (8192, 512), (4096, 3072), (8192, 3072))
+CANDIDATES = ((4096, 768), (2048, 768), (8192, 768), (16384, 768), (4096, 512), (8192, 512), (4096, 1536), (8192, 1536))
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 
 

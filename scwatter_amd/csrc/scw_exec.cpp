@@ -504,10 +504,10 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
 
     const int gx = (int)((6LL * nx + 255) / 256);
     // Each wave keeps its A-operand fragment (from U_z) in registers for all
-    // of its y tiles, so split y across workgroups only as far as needed to
-    // give every CU several workgroups (about 1536 in all: enough that the
-    // fragment loads of starting workgroups overlap the stores of running ones).
-    const int64_t mtarget = ex->tune_main_blocks > 0 ? ex->tune_main_blocks : tunable("SCW_MAIN_BLOCKS", 1536);
+    // of its y tiles, and the 16-bit workgroups compute their U_z columns
+    // themselves, so split y across workgroups only as far as needed to give
+    // every CU a few workgroups (about 768 in all).
+    const int64_t mtarget = ex->tune_main_blocks > 0 ? ex->tune_main_blocks : tunable("SCW_MAIN_BLOCKS", 768);
     const int ysplit =
         (int)std::min<int64_t>(ytiles, std::max<int64_t>(1, (mtarget + (int64_t)gx * nz - 1) / ((int64_t)gx * nz)));
     ex->ytiles_per_split = (ytiles + ysplit - 1) / ysplit;
