@@ -612,7 +612,7 @@ __device__ inline void xc_mfma(floatx4& acc, const bf16x8& ah, const bf16x8& al,
 // issues all its C loads at once: one L2 round trip per row tile instead of
 // one per K step; larger mx takes the K steps one at a time.
 template <int XS>
-__device__ inline void xcontract_columns(const ZMeta& zm, const RegionDev& R, const float2* __restrict__ Cs,
+__device__ __attribute__((always_inline)) inline void xcontract_columns(const ZMeta& zm, const RegionDev& R, const float2* __restrict__ Cs,
                                          const float2* __restrict__ X, int nx, float2 (*Ul)[UPL], XFrag* xs) {
     const int lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4, o = j & 1;
     const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
@@ -639,14 +639,22 @@ __device__ inline void xcontract_columns(const ZMeta& zm, const RegionDev& R, co
             }
     };
     if (XS > 0 && T <= XS) {  // uniform
-        for (int f = threadIdx.x; f < 3 * T * 64; f += 256) {
-            const int l = f & 63, ct = (f >> 6) % 3, t = (f >> 6) / 3;
+        // unrolled to the largest count (3 XS 64 / 256 per thread) so the
+        // loads of all iterations are in flight together; the loads are
+        // unconditional (index clamped), only the LDS writes are guarded
+        const int nf = 3 * T * 64;
+#pragma unroll
+        for (int it = 0; it < (3 * (XS > 0 ? XS : 1) * 64 + 255) / 256; ++it) {
+            const int f = threadIdx.x + 256 * it, fc = min(f, nf - 1);
+            const int l = fc & 63, ct = (fc >> 6) % 3, t = (fc >> 6) / 3;
             const int x = min(x_lo + 16 * ct + (l & 15), nx - 1);
             const floatx4* xp = reinterpret_cast<const floatx4*>(Xr + (int64_t)x * R.ka + 16 * t + 4 * (l >> 4));
             bf16x8 bh, bl;
             xc_b_operand(xp[0], xp[1], bh, bl);
-            xs[f][0] = __builtin_bit_cast(uint32x4, bh);
-            xs[f][1] = __builtin_bit_cast(uint32x4, bl);
+            if (f < nf) {
+                xs[f][0] = __builtin_bit_cast(uint32x4, bh);
+                xs[f][1] = __builtin_bit_cast(uint32x4, bl);
+            }
         }
         __syncthreads();
         for (int rt = wv; rt < RT; rt += 4) {
