@@ -235,6 +235,56 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
         return fail(SCW_ERR_UNSUPPORTED, "GEMM form: nx = %d is too large for %d distinct ky (U slice over 2 GiB)",
                     nx, kmax / 2);
 
+    // FP16 (5 exponent bits) cannot hold both factors of an evanescent mode:
+    // across a slab of height h, |exp(i kz (z - zref))| spans e^(+-Im(kz) h / 2)
+    // and the amplitude re-referenced to zref carries the inverse factor (e^12
+    // and e^-12 for a 4 um slab at 0.4 um).  Mode c of a dense region is
+    // therefore scaled by 2^-s_c in Z' and by 2^s_c in its amplitudes (an
+    // exec-owned copy of the plan's exponent plane), with s_c chosen so that
+    // the largest |Z_c| over the grid's z lies in [1/2, 1).
+    std::vector<std::vector<int32_t>> zshift(NV);
+    if (ex->dtype == SCW_DTYPE_FP16) {
+        int64_t words = 0, nshift = 0;
+        std::vector<int64_t> woff(NV, -1), soff(NV, -1);
+        for (int v = 0; v < NV; ++v) {
+            if (!rd[v].bfp) continue;
+            const RegionPlan& R = vreg[v].first->regions[vreg[v].second];
+            const int64_t groups = 4LL * ((R.width + 15) / 16);
+            std::vector<double> lmax(R.width, -HUGE_VAL);  // ln max |Z_c|
+            for (int z = 0; z < nz; ++z)
+                if (zv[z] == v)
+                    for (int c = 0; c < R.width; ++c)
+                        lmax[c] = std::max(lmax[c], -R.kz[c].imag() * (ze[z].z - R.zref));
+            zshift[v].assign((size_t)groups * 4, 0);
+            for (int c = 0; c < R.width; ++c)
+                zshift[v][c] = (int32_t)std::min(120.0, std::max(-120.0, std::floor(lmax[c] / M_LN2) + 1.0));
+            woff[v] = words;
+            words += groups * R.arow;
+            soff[v] = nshift;
+            nshift += groups * 4;
+        }
+        if (words > 0) {
+            int st2;
+            if ((st2 = ex->d_bfpx.alloc((size_t)(words + nshift) * 4)) != SCW_OK) return st2;
+            std::vector<int32_t> shifts((size_t)nshift);
+            for (int v = 0; v < NV; ++v)
+                if (soff[v] >= 0) std::copy(zshift[v].begin(), zshift[v].end(), shifts.begin() + soff[v]);
+            int32_t* dshift = ex->d_bfpx.as<int32_t>() + words;
+            SCW_HIP(hipMemcpy(dshift, shifts.data(), shifts.size() * 4, hipMemcpyHostToDevice));
+            for (int v = 0; v < NV; ++v) {
+                if (woff[v] < 0) continue;
+                const RegionPlan& R = vreg[v].first->regions[vreg[v].second];
+                const int64_t n = 4LL * ((R.width + 15) / 16) * R.arow;
+                uint8_t* dst = ex->d_bfpx.as<uint8_t>() + woff[v] * 4;
+                if ((st2 = shift_bfp_exponents(rd[v].bfp_exp, dst, R.arow, n, dshift + soff[v])) != SCW_OK)
+                    return st2;
+                rd[v].bfp_exp = dst;
+            }
+        } else {
+            ex->d_bfpx.release();
+        }
+    }
+
     // Runs: consecutive z of one virtual region.  Dense eigenmode regions
     // (every cell holds every kz, term index = kz index) contract on MFMA in
     // runs of up to 64 z, every term read once per run; other regions use the
@@ -382,7 +432,8 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
                         if (!region_ok(val))
                             return fail(SCW_ERR_UNSUPPORTED, "region %d: exp(i kz (z - zref)) outside FP32 range at z=%g",
                                         vreg[rn.v].second, ze[rn.z0 + zi].z);
-                        zc[(size_t)zi * d.width + c] = val;
+                        zc[(size_t)zi * d.width + c] =
+                            zshift[rn.v].empty() ? val : val * std::ldexp(1.0, -zshift[rn.v][c]);
                     }
                 zf.resize(zf.size() + (size_t)MT * T * 64 * 8);
                 uint32_t* out = zf.data() + it.zf_off;
@@ -552,7 +603,7 @@ static void exec_free(scw_exec* ex) {
     if (ex->graph) (void)hipGraphDestroy(ex->graph);
     if (ex->graph_stream) (void)hipStreamDestroy(ex->graph_stream);
     for (DevBuf* b : {&ex->d_dx, &ex->d_dy, &ex->d_z, &ex->d_zslab, &ex->d_items, &ex->d_mitems, &ex->d_regions, &ex->d_zmeta,
-                      &ex->d_Z, &ex->d_Zf, &ex->d_X, &ex->d_C, &ex->d_U, &ex->d_Yfrag, &ex->d_out})
+                      &ex->d_Z, &ex->d_Zf, &ex->d_X, &ex->d_C, &ex->d_U, &ex->d_Yfrag, &ex->d_bfpx, &ex->d_out})
         b->release();
     delete ex;
 }

@@ -961,6 +961,34 @@ int build_bfp(scw_plan* P) {
     return SCW_OK;
 }
 
+// Exponent plane of a dense region with mode 4 q + h scaled by 2^shift[4 q + h]
+// (dst[i] = src[i] with each byte E -> clamp(E + shift, 1, 255); byte 1 reads
+// as scale 0).  The FP16 mode contraction moves part of each mode's
+// magnitude from its phase factor into its amplitudes this way.
+__global__ __launch_bounds__(256) void bfp_shift_exp(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                                                     int64_t arow, int64_t n, const int32_t* __restrict__ shift) {
+    for (int64_t i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int64_t q = i / arow;
+        const uint32_t w = src[i];
+        uint32_t o = 0;
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            const int e = min(max((int)((w >> (8 * h)) & 0xffu) + shift[4 * q + h], 1), 255);
+            o |= (uint32_t)e << (8 * h);
+        }
+        dst[i] = o;
+    }
+}
+
+int shift_bfp_exponents(const uint8_t* src, uint8_t* dst, int64_t arow, int64_t n, const int32_t* shift) {
+    const dim3 g((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536)));
+    hipLaunchKernelGGL(bfp_shift_exp, g, dim3(256), 0, nullptr, reinterpret_cast<const uint32_t*>(src),
+                       reinterpret_cast<uint32_t*>(dst), arow, n, shift);
+    SCW_HIP(hipGetLastError());
+    SCW_HIP(hipDeviceSynchronize());
+    return SCW_OK;
+}
+
 // ---------------------------------------------------------------------------
 // MFMA mode contraction for dense eigenmode regions (every cell holds every
 // kz; the term index is the kz index c):

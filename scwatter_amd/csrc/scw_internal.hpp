@@ -173,6 +173,7 @@ struct scw_exec {
     std::vector<scw::ModalItem> mitems;   // grouped by row-tile class (MT = 1, 2, 4, 8)
     int64_t mclass_off[5] = {0, 0, 0, 0, 0};
     scw::DevBuf d_items, d_mitems, d_regions, d_zmeta, d_Z, d_Zf, d_X, d_C, d_U, d_Yfrag;
+    scw::DevBuf d_bfpx;       // FP16: shifted exponent planes of the dense regions, then the shifts
     int64_t tune_coef_items = 0;   // launch-shape choices (0 = default)
     int64_t tune_main_blocks = 0;
     std::vector<double> hx, hy;    // grid kept for re-planning launch shapes
@@ -195,4 +196,7 @@ namespace scw {
 int launch_direct(const scw_exec* ex, float* out, hipStream_t s);
 int launch_gemm(const scw_exec* ex, float* out, hipStream_t s, int stage = 0);
 int build_bfp(scw_plan* plan);  // fills plan->d_bfp / bfp_off once (scw_gemm.hip)
+// dst = the exponent plane src (n records, arow per group) with the byte of
+// mode c raised by shift[c] (device pointers; scw_gemm.hip)
+int shift_bfp_exponents(const uint8_t* src, uint8_t* dst, int64_t arow, int64_t n, const int32_t* shift);
 }  // namespace scw

@@ -153,6 +153,25 @@ def test_dense_slab_mode_contraction(slab_dense, dtype, monkeypatch):
     _check(Exec(plan, *g, form="gemm", dtype=dtype).run_host(), ref, TOL[dtype], f"dense slab VALU {dtype}")
 
 
+@pytest.mark.parametrize("dtype,tol", [("fp16", 2e-3), ("bf16x3", 1e-4)])
+def test_strongly_evanescent_slab(dtype, tol):
+    """A 4 um slab at 0.4 um with evanescent modes up to Im(kz) = 6.3 / um:
+    across the slab exp(i kz (z - zref)) spans e^(+-12.6), past FP16's range,
+    and the amplitudes re-referenced to the slab's middle carry the inverse
+    factor.  The FP16 mode contraction moves a power of two per mode from the
+    phase factor into the amplitudes; without it FP16 lost the weak factors
+    (rel-L2 about 7e-3 on the 64-slab batch, worse than BF16)."""
+    cw = configs.tile_sample_structure(M=5, Z=1, size=(12.0, 12.0, 4.0), lam=0.4, seed=11,
+                                       evanescent_fraction=0.3, incidence_deg=10.0)
+    t = cw.unpack()
+    plan = Plan(t)
+    kz = t.W[t.begin[1]:t.end[1], 5]
+    assert np.abs(kz.imag).max() * 2.0 > 10.0  # |Z| past e^10 at the slab faces
+    xs, ys, zs = configs.axis(-6, 6, 16), configs.axis(-6, 6, 14), configs.axis(-1.95, 1.95, 40)
+    ref = oracle.eval_grid_f64(t.W, t.z_layers, t.begin, t.end, xs, ys, zs)
+    _check(Exec(plan, xs, ys, zs, form="gemm", dtype=dtype).run_host(), ref, tol, f"evanescent slab {dtype}")
+
+
 @pytest.mark.parametrize("dtype", ["bf16x3", "bf16", "fp32"])
 def test_wide_pupil(dtype):
     """51 x 51 orders: more than 48 kx and ky per region, so the 16-bit main
