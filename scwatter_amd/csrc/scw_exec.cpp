@@ -172,7 +172,7 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
         // kernel; x-major [nx][ka] (zero past mx) for the 16-bit dtypes, whose
         // main GEMM reads four consecutive a of one x per lane
         d.ka = (R.mx + 15) / 16 * 16;
-        d.pad_ = 0;
+        d.erow = (int32_t)bfp_erow(R.arow);
         d.x_off = (int64_t)X.size() / 2;
         X.resize(X.size() + (size_t)2 * (is16 ? (int64_t)d.ka : R.mx) * nx, 0.0f);
         for (int a = 0; a < R.mx; ++a)
@@ -259,7 +259,7 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
             for (int c = 0; c < R.width; ++c)
                 zshift[v][c] = (int32_t)std::min(120.0, std::max(-120.0, std::floor(lmax[c] / M_LN2) + 1.0));
             woff[v] = words;
-            words += groups * R.arow;
+            words += groups * bfp_erow(R.arow);
             soff[v] = nshift;
             nshift += groups * 4;
         }
@@ -274,9 +274,9 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
             for (int v = 0; v < NV; ++v) {
                 if (woff[v] < 0) continue;
                 const RegionPlan& R = vreg[v].first->regions[vreg[v].second];
-                const int64_t n = 4LL * ((R.width + 15) / 16) * R.arow;
+                const int64_t n = 4LL * ((R.width + 15) / 16) * bfp_erow(R.arow);
                 uint8_t* dst = ex->d_bfpx.as<uint8_t>() + woff[v] * 4;
-                if ((st2 = shift_bfp_exponents(rd[v].bfp_exp, dst, R.arow, n, dshift + soff[v])) != SCW_OK)
+                if ((st2 = shift_bfp_exponents(rd[v].bfp_exp, dst, bfp_erow(R.arow), n, dshift + soff[v])) != SCW_OK)
                     return st2;
                 rd[v].bfp_exp = dst;
             }
@@ -399,10 +399,10 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
             r0 = r1;
             continue;
         }
-        // terms once per run: 24 bytes (3 complex FP32), or 15 for the 16-bit
+        // terms once per run: 24 bytes (3 complex FP32), or 13 for the 16-bit
         // mode contraction's block floating-point amplitudes (3 columns of 2
-        // 16-bit mantissas and an exponent byte); plus the Z rows
-        const double term_bytes = runs[r0].modal && is16 ? 15.0 : 24.0;
+        // 16-bit mantissas and one exponent byte); plus the Z rows
+        const double term_bytes = runs[r0].modal && is16 ? 13.0 : 24.0;
         for (size_t r = r0; r < r1; ++r)
             b_coef += term_bytes * (double)d.cells * d.width + 8.0 * d.mc * (runs[r].z1 - runs[r].z0);
         if (runs[r0].modal) {

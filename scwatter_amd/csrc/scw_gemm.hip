@@ -840,48 +840,64 @@ static void launch_main(const scw_exec* ex, dim3 grid, hipStream_t st, int ytps,
 
 // ---------------------------------------------------------------------------
 // Block floating-point amplitudes for the 16-bit dtypes.  Group q of a dense
-// region holds modes 4q .. 4q + 3 of every column m; record (q, m) is the 8
-// values v[e], e = 2 h + (re|im) of mode 4q + h, as 16-bit signed mantissas,
-// the re/im pair of each mode sharing one exponent byte:
-// v[e] = mant[e] 2^(E[h] - 128), E[h] chosen so that the larger part has 15
-// significant bits.  Every value then has at most 15 significant bits and
-// splits exactly into bf16 hi (its top 16 bits) + bf16 lo (the rest); its
-// error is at most 2^-15 of its mode's |re| or |im| (half a step of the
-// 15-bit scale), so each term keeps its own relative precision whatever the
-// magnitudes of the other modes (an exponent shared across modes loses it
-// for weak modes whose phase factor is large: rel-L2 3.4e-4 on the 64-slab
-// batch).  Planes: mantissas [q][arow] x 16 bytes (a 16-lane load of 16
-// columns reads 256 contiguous bytes), then exponents [q][arow] x 4 bytes.
-// 20 bytes per record against 24 for FP32 values rounded to 24 bits.  Modes
-// past the width and columns past 3 cells are zero.
+// region holds modes 4q .. 4q + 3 of every column m = 3 cell + comp; record
+// (q, m) is the 8 values v[e], e = 2 h + (re|im) of mode 4q + h, as 16-bit
+// signed mantissas, and the six values of one mode at one cell (3 components,
+// re and im) share one exponent byte: v[e] = mant[e] 2^(E[h] - 128), E[h]
+// chosen so that the largest of the six has 15 significant bits.  Every value
+// then has at most 15 significant bits and splits exactly into bf16 hi (its
+// top 16 bits) + bf16 lo (the rest); its error is at most 2^-15 of the
+// largest |re| or |im| of its term (half a step of the 15-bit scale), so each
+// term keeps its own relative precision whatever the magnitudes of the other
+// modes (an exponent shared across modes loses it for weak modes whose phase
+// factor is large: rel-L2 3.4e-4 on the 64-slab batch).  Planes: mantissas
+// [q][arow] x 16 bytes (a 16-lane load of 16 columns reads 256 contiguous
+// bytes), then exponents [q][erow] x 4 bytes (erow = cells, padded: arow / 3
+// rounded up).  13 bytes per term (3 complex values) against 24 for FP32
+// values.  Modes past the width and columns past 3 cells are zero.
 // ---------------------------------------------------------------------------
 constexpr int kBfpBias = 128;
 
-__global__ __launch_bounds__(256) void bfp_encode(const float2* __restrict__ A, int64_t arow, int width, int mcols,
-                                                  int64_t groups, uint32x4* __restrict__ mant,
+__global__ __launch_bounds__(256) void bfp_encode(const float2* __restrict__ A, int64_t arow, int64_t erow, int width,
+                                                  int mcols, int64_t groups, uint32x4* __restrict__ mant,
                                                   uint32_t* __restrict__ expo) {
-    const int64_t n = groups * arow;
+    const int64_t n = groups * erow;
     for (int64_t i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-        const int64_t q = i / arow, m = i - q * arow;
-        uint32x4 w;
+        const int64_t q = i / erow, cell = i - q * erow;
+        float2 a[4][3];
+        int s[4];
         uint32_t ex = 0;
 #pragma unroll
         for (int h = 0; h < 4; ++h) {
             const int64_t c = 4 * q + h;
-            const float2 a = (c < width && m < mcols) ? A[c * arow + m] : make_float2(0.0f, 0.0f);
-            const float mx = fmaxf(fabsf(a.x), fabsf(a.y));
-            // scale 2^s with mx 2^-s in [2^14, 2^15) (one step up if it rounds
-            // to 2^15), kept a normal float; a zero mode takes s = 0
-            int s = 0;
-            if (mx > 0.0f) {
-                s = max(ilogbf(mx) - 14, -126);
-                if (rintf(ldexpf(mx, -s)) > 32767.0f) ++s;
+            float mx = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const int64_t m = 3 * cell + k;
+                a[h][k] = (c < width && m < mcols) ? A[c * arow + m] : make_float2(0.0f, 0.0f);
+                mx = fmaxf(mx, fmaxf(fabsf(a[h][k].x), fabsf(a[h][k].y)));
             }
-            const int re = (int)rintf(ldexpf(a.x, -s)), im = (int)rintf(ldexpf(a.y, -s));
-            w[h] = ((uint32_t)re & 0xffffu) | ((uint32_t)im << 16);
-            ex |= (uint32_t)(s + kBfpBias) << (8 * h);
+            // scale 2^s with mx 2^-s in [2^14, 2^15) (one step up if it rounds
+            // to 2^15), kept a normal float; a zero term takes s = 0
+            s[h] = 0;
+            if (mx > 0.0f) {
+                s[h] = max(ilogbf(mx) - 14, -126);
+                if (rintf(ldexpf(mx, -s[h])) > 32767.0f) ++s[h];
+            }
+            ex |= (uint32_t)(s[h] + kBfpBias) << (8 * h);
         }
-        mant[i] = w;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int64_t m = 3 * cell + k;
+            if (m >= arow) break;  // the last cell of a padded row
+            uint32x4 w;
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                const int re = (int)rintf(ldexpf(a[h][k].x, -s[h])), im = (int)rintf(ldexpf(a[h][k].y, -s[h]));
+                w[h] = ((uint32_t)re & 0xffffu) | ((uint32_t)im << 16);
+            }
+            mant[q * arow + m] = w;
+        }
         expo[i] = ex;
     }
 }
@@ -898,13 +914,15 @@ __device__ inline float bfp_scale(uint32_t e) { return __uint_as_float((e - (kBf
 
 // One lane's record: the 4 mantissa words and the exponent word.
 typedef uint32_t RawBfp[5];
+// Record (q, column): mantissas mrow[column] and exponents erow[column / 3]
+// of mode group q's rows mrow = mantissas + q arow, erow = exponents + q erow.
 template <bool NT>
-__device__ inline void bfp_load(RawBfp& r, const RegionDev& R, int64_t rec) {
-    const uint32x4* mp = reinterpret_cast<const uint32x4*>(R.bfp) + rec;
+__device__ inline void bfp_load(RawBfp& r, const uint32x4* mrow, const uint32_t* erow, uint32_t col, uint32_t cell) {
+    const uint32x4* mp = mrow + col;
     const uint32x4 w = NT ? stream_load(mp) : gload(mp);
 #pragma unroll
     for (int k = 0; k < 4; ++k) r[k] = w[k];
-    const uint32_t* ep = reinterpret_cast<const uint32_t*>(R.bfp_exp) + rec;
+    const uint32_t* ep = erow + cell;
     r[4] = NT ? stream_load(ep) : gload(ep);
 }
 // 16-bit MFMA operands of a record: MODE 1 the exact hi/lo split, MODE 0 / 2
@@ -939,7 +957,7 @@ int build_bfp(scw_plan* P) {
         if (!R.separable || !R.dense || R.width == 0) continue;
         off[r] = bytes;
         const int64_t groups = 4LL * ((R.width + 15) / 16);
-        bytes += groups * R.arow * 20;
+        bytes += groups * (R.arow * 16 + bfp_erow(R.arow) * 4);
         bytes = (bytes + 255) / 256 * 256;  // next region's mantissas 256-byte aligned
     }
     int st;
@@ -948,12 +966,12 @@ int build_bfp(scw_plan* P) {
         if (off[r] < 0) continue;
         const RegionPlan& R = P->regions[r];
         const int64_t groups = 4LL * ((R.width + 15) / 16);
-        const int64_t n = groups * R.arow;
+        const int64_t n = groups * bfp_erow(R.arow);  // one thread per (mode group, cell)
         uint8_t* base = P->d_bfp.as<uint8_t>() + off[r];
         const dim3 g((unsigned)std::min<int64_t>((n + 255) / 256, 65536));
         hipLaunchKernelGGL(bfp_encode, g, dim3(256), 0, nullptr, P->d_ellA.as<float2>() + R.ell_off, R.arow,
-                           R.width, (int)(3 * (int64_t)R.mx * R.my), groups, reinterpret_cast<uint32x4*>(base),
-                           reinterpret_cast<uint32_t*>(base + n * 16));
+                           bfp_erow(R.arow), R.width, (int)(3 * (int64_t)R.mx * R.my), groups,
+                           reinterpret_cast<uint32x4*>(base), reinterpret_cast<uint32_t*>(base + groups * R.arow * 16));
         SCW_HIP(hipGetLastError());
     }
     SCW_HIP(hipDeviceSynchronize());
@@ -966,9 +984,9 @@ int build_bfp(scw_plan* P) {
 // as scale 0).  The FP16 mode contraction moves part of each mode's
 // magnitude from its phase factor into its amplitudes this way.
 __global__ __launch_bounds__(256) void bfp_shift_exp(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
-                                                     int64_t arow, int64_t n, const int32_t* __restrict__ shift) {
+                                                     int64_t erow, int64_t n, const int32_t* __restrict__ shift) {
     for (int64_t i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-        const int64_t q = i / arow;
+        const int64_t q = i / erow;
         const uint32_t w = src[i];
         uint32_t o = 0;
 #pragma unroll
@@ -980,10 +998,10 @@ __global__ __launch_bounds__(256) void bfp_shift_exp(const uint32_t* __restrict_
     }
 }
 
-int shift_bfp_exponents(const uint8_t* src, uint8_t* dst, int64_t arow, int64_t n, const int32_t* shift) {
+int shift_bfp_exponents(const uint8_t* src, uint8_t* dst, int64_t erow, int64_t n, const int32_t* shift) {
     const dim3 g((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536)));
     hipLaunchKernelGGL(bfp_shift_exp, g, dim3(256), 0, nullptr, reinterpret_cast<const uint32_t*>(src),
-                       reinterpret_cast<uint32_t*>(dst), arow, n, shift);
+                       reinterpret_cast<uint32_t*>(dst), erow, n, shift);
     SCW_HIP(hipGetLastError());
     SCW_HIP(hipDeviceSynchronize());
     return SCW_OK;
@@ -1035,9 +1053,12 @@ __device__ inline void modal_body(const ModalItem& I, const RegionDev& R, const 
     const int mcols = 3 * R.cells;
     const int W = R.width;
     const float2* A = R.ellA;
-    int col[4];
+    int col[4], ecol[4];  // column and its cell (block floating-point exponent word)
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt) col[nt] = min(I.m0 + 16 * nt + j, mcols - 1);
+    for (int nt = 0; nt < 4; ++nt) {
+        col[nt] = min(I.m0 + 16 * nt + j, mcols - 1);
+        ecol[nt] = col[nt] / 3;
+    }
     floatx4 acc[4][MT];
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt)
@@ -1066,9 +1087,10 @@ __device__ inline void modal_body(const ModalItem& I, const RegionDev& R, const 
                 for (int nt = 0; nt < 4; ++nt) ra[nt][h] = MT >= 4 ? stream_load(row + col[nt]) : gload(row + col[nt]);
             }
         } else {
-            const int64_t grp = (int64_t)(4 * t + g) * R.arow;
+            const uint32x4* mrow = reinterpret_cast<const uint32x4*>(R.bfp) + (int64_t)(4 * t + g) * R.arow;
+            const uint32_t* erow = reinterpret_cast<const uint32_t*>(R.bfp_exp) + (int64_t)(4 * t + g) * R.erow;
 #pragma unroll
-            for (int nt = 0; nt < 4; ++nt) bfp_load<(MT >= 4)>(ra[nt], R, grp + col[nt]);
+            for (int nt = 0; nt < 4; ++nt) bfp_load<(MT >= 4)>(ra[nt], mrow, erow, col[nt], ecol[nt]);
         }
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
@@ -1151,9 +1173,12 @@ __device__ inline void modal_body_wg(const ModalItem& I, const ModalItem& L, con
     const int lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4;
     const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const int mcols = 3 * R.cells;
-    int col[4];
+    int col[4], ecol[4];  // column and its cell (block floating-point exponent word)
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt) col[nt] = min(I.m0 + 16 * nt + j, mcols - 1);
+    for (int nt = 0; nt < 4; ++nt) {
+        col[nt] = min(I.m0 + 16 * nt + j, mcols - 1);
+        ecol[nt] = col[nt] / 3;
+    }
     floatx4 acc[4][MT];
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt)
@@ -1165,9 +1190,10 @@ __device__ inline void modal_body_wg(const ModalItem& I, const ModalItem& L, con
     // steps, columns past the table re-read the last column, and prefetches
     // past the last step re-read it.
     auto loadA = [&](RawB& ra, int t) {
-        const int64_t grp = (int64_t)(4 * t + g) * R.arow;
+        const uint32x4* mrow = reinterpret_cast<const uint32x4*>(R.bfp) + (int64_t)(4 * t + g) * R.arow;
+        const uint32_t* erow = reinterpret_cast<const uint32_t*>(R.bfp_exp) + (int64_t)(4 * t + g) * R.erow;
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) bfp_load<true>(ra[nt], R, grp + col[nt]);
+        for (int nt = 0; nt < 4; ++nt) bfp_load<true>(ra[nt], mrow, erow, col[nt], ecol[nt]);
     };
     // This wave's quarter of a step's fragments: row tiles 2 wv, 2 wv + 1.
     // The steps and fragments are the workgroup leader's L (the same run and

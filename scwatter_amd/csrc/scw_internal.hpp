@@ -85,10 +85,10 @@ struct scw_plan {
     scw::DevBuf d_ellC;  // int32[...][width][cells]      kz index (sparse regions only)
     int64_t ell_elems = 0;
     // Dense-region amplitudes for the 16-bit GEMM dtypes in block floating
-    // point: per group of 4 modes and column, 8 16-bit mantissas and an
-    // exponent byte per mode, [region]{[mode / 4][arow] x 16 bytes, then
-    // [mode / 4][arow] x 4 bytes}.  Built on the first 16-bit exec that needs
-    // it (scw::build_bfp).
+    // point: per group of 4 modes and column, 8 16-bit mantissas; per group
+    // and cell, an exponent byte per mode, [region]{[mode / 4][arow] x 16
+    // bytes, then [mode / 4][bfp_erow(arow)] x 4 bytes}.  Built on the first
+    // 16-bit exec that needs it (scw::build_bfp).
     scw::DevBuf d_bfp;
     std::vector<int64_t> bfp_off;  // byte offset per region, -1 if none
 };
@@ -137,10 +137,15 @@ struct RegionDev {      // per-region data the GEMM-form kernels read (device)
     int64_t x_off;      // float2 offset of X_r: [mx][nx] (FP32) or [nx][ka] (16-bit dtypes, zero past mx)
     int32_t cells, width, mc, dense;
     int32_t mx, my, nslots, arow;
-    int32_t ka, pad_;   // mx rounded up to 16 (the X contraction's K in complex steps)
+    int32_t ka;         // mx rounded up to 16 (the X contraction's K in complex steps)
+    int32_t erow;       // exponent words per mode group of the block floating-point table: bfp_erow(arow)
     const uint8_t* bfp;      // block floating-point mantissas of a dense region (16-bit dtypes), else nullptr
-    const uint8_t* bfp_exp;  // their exponent bytes (4 per record)
+    const uint8_t* bfp_exp;  // their exponents: one word per (mode group, cell), one byte per mode
 };
+
+// Cells per mode group of the block floating-point exponent plane (the
+// columns of a term row, arow, are 3 per cell, padded).
+inline int64_t bfp_erow(int64_t arow) { return (arow + 2) / 3; }
 
 // Direct form helpers (scw_direct.hip).
 size_t direct_region_bytes();
@@ -196,7 +201,7 @@ namespace scw {
 int launch_direct(const scw_exec* ex, float* out, hipStream_t s);
 int launch_gemm(const scw_exec* ex, float* out, hipStream_t s, int stage = 0);
 int build_bfp(scw_plan* plan);  // fills plan->d_bfp / bfp_off once (scw_gemm.hip)
-// dst = the exponent plane src (n records, arow per group) with the byte of
-// mode c raised by shift[c] (device pointers; scw_gemm.hip)
-int shift_bfp_exponents(const uint8_t* src, uint8_t* dst, int64_t arow, int64_t n, const int32_t* shift);
+// dst = the exponent plane src (n words, erow per mode group) with the byte
+// of mode c raised by shift[c] (device pointers; scw_gemm.hip)
+int shift_bfp_exponents(const uint8_t* src, uint8_t* dst, int64_t erow, int64_t n, const int32_t* shift);
 }  // namespace scw
