@@ -11,6 +11,9 @@ fixes, chosen so a plausible mistake fails at least one check:
     - numpy.fft.ifft2 on a uniform lattice (a textbook routine): catches
       transposed operands on a non-square grid and a wrong sign convention;
     - linearity (superposition) and translation (wind) invariants;
+    - closed forms of a single wave: constant |E| and a phase step of k.dr for
+      real k, exp(-Im(kz) z) decay in an absorbing medium;
+    - a long-double term-by-term sum on tiny inputs (16 waves, 5x4x3 points);
     - region membership on exact integer-valued fields.
   refcpu_eval_{xy,yz,xz,grid} (FP32 reference-semantics restatement):
     - agreement with the pinned FP64 oracle at the same float coordinates
@@ -88,6 +91,51 @@ def test_translation():
     Wt[:, :3] *= np.exp(1j * (W[:, 3:] @ t))[:, None]
     f = lambda W, a, b, c: oracle.eval_grid_f64(W, np.zeros(0), [0], [W.shape[0]], a, b, c)
     assert _rel(f(W, xs + t[0], ys + t[1], zs + t[2]), f(Wt, xs, ys, zs)) < 1e-12
+
+
+def test_single_wave_closed_form():
+    """One wave gives E0 exp(i k.r) exactly: for real k, |E_c| = |E0_c| at every
+    point and the phase advances by k.dr between points; for an absorbing
+    medium (Im kz > 0), |E_c(z)| = |E0_c| exp(-Im(kz) z)."""
+    rng = np.random.default_rng(5)
+    xs, ys, zs = np.linspace(-4, 4, 7), np.linspace(-3, 5, 5), np.linspace(-2, 3, 6)
+    E0 = rng.standard_normal(3) + 1j * rng.standard_normal(3)
+    k = np.array([1.3, -0.7, 2.1])
+    W = np.concatenate([E0, k]).astype(np.complex128)[None, :]
+    E = oracle.eval_grid_f64(W, np.zeros(0), [0], [1], xs, ys, zs)  # [z][y][x][3]
+    assert np.allclose(np.abs(E), np.abs(E0)[None, None, None, :], rtol=1e-13, atol=0)
+    ratio = E[:, :, 1:, :] / E[:, :, :-1, :]
+    assert np.allclose(ratio, np.exp(1j * k[0] * (xs[1] - xs[0])), rtol=1e-12, atol=0)
+    kz = 2.1 + 0.35j
+    W[0, 5] = kz
+    E = oracle.eval_grid_f64(W, np.zeros(0), [0], [1], xs, ys, zs)
+    amp = np.abs(E0)[None, :] * np.exp(-kz.imag * zs)[:, None]
+    assert np.allclose(np.abs(E[:, 2, 3, :]), amp, rtol=1e-13, atol=0)
+
+
+def test_brute_force_long_double():
+    """Tiny inputs (W <= 16, N <= 8) against a long-double sum written out
+    term by term (numpy longdouble), independent of the oracle's code path."""
+    rng = np.random.default_rng(6)
+    W = _rand_waves(rng, 16)
+    xs, ys, zs = rng.uniform(-2, 2, 5), rng.uniform(-2, 2, 4), rng.uniform(-2, 2, 3)
+    E = oracle.eval_grid_f64(W, np.zeros(0), [0], [W.shape[0]], xs, ys, zs)
+    ld = np.longdouble
+    for iz, z in enumerate(zs):
+        for iy, y in enumerate(ys):
+            for ix, x in enumerate(xs):
+                acc = np.zeros((3, 2), ld)
+                for w in W:
+                    kr = (ld(w[3].real) * ld(x) + ld(w[4].real) * ld(y) + ld(w[5].real) * ld(z))
+                    ki = (ld(w[3].imag) * ld(x) + ld(w[4].imag) * ld(y) + ld(w[5].imag) * ld(z))
+                    mag = np.exp(-ki)
+                    c, s = np.cos(kr) * mag, np.sin(kr) * mag
+                    for comp in range(3):
+                        er, ei = ld(w[comp].real), ld(w[comp].imag)
+                        acc[comp, 0] += er * c - ei * s
+                        acc[comp, 1] += er * s + ei * c
+                ref = acc[:, 0].astype(np.float64) + 1j * acc[:, 1].astype(np.float64)
+                assert np.allclose(E[iz, iy, ix], ref, rtol=1e-12, atol=1e-13 * np.abs(ref).max())
 
 
 @pytest.mark.parametrize("tol", [0.0, 1e-3])
