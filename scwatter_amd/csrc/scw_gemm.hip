@@ -152,6 +152,19 @@ __device__ inline void coef_loop(float (&acc)[8][6], const float2* __restrict__ 
     }
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
+// XCD b % 8), each with its own L2.  The mode contraction (modal_kernel) uses
+// this logical block index instead of blockIdx.x: XCD x takes the contiguous
+// logical range [x q + min(x, r), ...) (q = n / 8, r = n % 8, a bijection on
+// [0, n)), so the workgroups of one run and mode slot, which read the same
+// phase fragments, share an L2 instead of each XCD fetching them from HBM.
+// Tile sample, BF16x3: mode contraction 43.0 -> 38.2 us.  (The VALU item
+// kernel and the 64-z workgroup kernel measured no gain from it.)
+__device__ inline int xcd_block() {
+    const int nb = (int)gridDim.x, b = (int)blockIdx.x, q = nb >> 3, r = nb & 7, x = b & 7;
+    return x * q + min(x, r) + (b >> 3);
+}
+
 __global__ __launch_bounds__(256) void coef_kernel(const CoefItem* __restrict__ items, int nitems,
                                                    const RegionDev* __restrict__ regions,
                                                    const ZMeta* __restrict__ zmeta, const float2* __restrict__ Z,
@@ -1265,7 +1278,7 @@ __global__ __launch_bounds__(256) void modal_kernel(const ModalItem* __restrict_
                                                     const RegionDev* __restrict__ regions,
                                                     const ZMeta* __restrict__ zmeta, const uint32_t* __restrict__ zf,
                                                     int64_t c_elems, float2* __restrict__ C) {
-    const int it = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+    const int it = __builtin_amdgcn_readfirstlane(xcd_block() * 4 + (int)(threadIdx.x >> 6));
     if (it >= nitems) return;
     const ModalItem I = items[it];
     modal_body<MODE, MT>(I, regions[I.region], zmeta, zf, c_elems, C);
