@@ -164,6 +164,18 @@ __device__ inline int xcd_block() {
     const int nb = (int)gridDim.x, b = (int)blockIdx.x, q = nb >> 3, r = nb & 7, x = b & 7;
     return x * q + min(x, r) + (b >> 3);
 }
+// The same order over a 3D grid (dispatch order: x fastest, then y, then z),
+// returned as logical (x, y, z) block coordinates.
+__device__ inline void xcd_block3(int& bx, int& by, int& bz) {
+    const int gx = (int)gridDim.x, gy = (int)gridDim.y;
+    const int nb = gx * gy * (int)gridDim.z;
+    const int b = (int)blockIdx.x + gx * ((int)blockIdx.y + gy * (int)blockIdx.z);
+    const int q = nb >> 3, r = nb & 7, x = b & 7;
+    const int l = x * q + min(x, r) + (b >> 3);
+    bx = l % gx;
+    by = (l / gx) % gy;
+    bz = l / (gx * gy);
+}
 
 __global__ __launch_bounds__(256) void coef_kernel(const CoefItem* __restrict__ items, int nitems,
                                                    const RegionDev* __restrict__ regions,
@@ -615,9 +627,10 @@ __device__ inline void xc_mfma(floatx4& acc, const bf16x8& ah, const bf16x8& al,
     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
 }
 
-// U_z[b][128 blockIdx.x + pl] for pl < 128 and b < my into Ul[b][pl].  Wave
+// U_z[b][128 bx + pl] for pl < 128 and b < my into Ul[b][pl] (bx: the
+// workgroup's column block).  Wave
 // wv takes row tiles wv, wv + 4, ... (8 rows m each) against the 3 column
-// tiles of 16 x from x_lo = 128 blockIdx.x / 3 on (pl = 3 (x - x_lo) + comp -
+// tiles of 16 x from x_lo = 128 bx / 3 on (pl = 3 (x - x_lo) + comp -
 // r0 covers 0..127 with x - x_lo < 44).  Rows past 3 my and x past nx
 // re-read the last row / column (results not stored / only for columns the
 // stores drop).  With at most XS K steps (mx <= 16 XS) the workgroup
@@ -626,10 +639,10 @@ __device__ inline void xc_mfma(floatx4& acc, const bf16x8& ah, const bf16x8& al,
 // one per K step; larger mx takes the K steps one at a time.
 template <int XS>
 __device__ __attribute__((always_inline)) inline void xcontract_columns(const ZMeta& zm, const RegionDev& R, const float2* __restrict__ Cs,
-                                         const float2* __restrict__ X, int nx, float2 (*Ul)[UPL], XFrag* xs) {
+                                         const float2* __restrict__ X, int nx, int bx, float2 (*Ul)[UPL], XFrag* xs) {
     const int lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4, o = j & 1;
     const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-    const int p0 = 128 * (int)blockIdx.x, x_lo = p0 / 3, r0 = p0 - 3 * x_lo;
+    const int p0 = 128 * bx, x_lo = p0 / 3, r0 = p0 - 3 * x_lo;
     const int M = 3 * zm.my, RT = (M + 7) >> 3, T = R.ka >> 4;
     const float2* Cz = Cs + zm.u_off;
     const float2* Xr = X + R.x_off;
@@ -729,19 +742,24 @@ __global__ __launch_bounds__(256) void main_16(const ZMeta* __restrict__ zmeta, 
     typedef typename Op16<MODE>::vec vec;
     typedef typename Op16<MODE>::elem elem;
     constexpr int S = KMAX / 32;
-    const int z = blockIdx.y + zoff;
+    // logical block (column block bx, z, y split) in XCD-contiguous order: the
+    // column blocks of one z and the z of one region share an XCD's L2 for
+    // C_z, X and the Y fragments
+    int bx, by, bz;
+    xcd_block3(bx, by, bz);
+    const int z = by + zoff;
     const ZMeta zm = zmeta[z];
     const int K32 = zm.kp >> 5;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t ncols = 6LL * nx;
-    const int64_t n0 = (int64_t)blockIdx.x * 256 + wv * 64;  // waves past ncols compute zeros
+    const int64_t n0 = (int64_t)bx * 256 + wv * 64;  // waves past ncols compute zeros
     const int li = lane & 15, lk = lane >> 4;
     const int ytiles = (ny + 15) >> 4;
-    const int yt0 = blockIdx.z * ytiles_per_split;
+    const int yt0 = bz * ytiles_per_split;
     const int yt1 = min(ytiles, yt0 + ytiles_per_split);
     float* oz = out + (int64_t)z * ny * ncols;
     if (K32 == 0) {
-        zero_tiles<VEC>(oz, ncols, ny, yt0, yt1, (int64_t)blockIdx.x * 256);
+        zero_tiles<VEC>(oz, ncols, ny, yt0, yt1, (int64_t)bx * 256);
         return;
     }
     // LDS: the workgroup's U_z columns while the fragments are built, then the
@@ -753,7 +771,7 @@ __global__ __launch_bounds__(256) void main_16(const ZMeta* __restrict__ zmeta, 
     float (*T)[16][TROW] = reinterpret_cast<float (*)[16][TROW]>(lds);
     float2 (*Ul)[UPL] = reinterpret_cast<float2 (*)[UPL]>(lds);
     XFrag* xs = reinterpret_cast<XFrag*>(lds + kUBytes);
-    xcontract_columns<kXS>(zm, regions[zm.region], Cs, X, nx, Ul, xs);
+    xcontract_columns<kXS>(zm, regions[zm.region], Cs, X, nx, bx, Ul, xs);
     __syncthreads();
 
     // A fragment: Bp[k][n] from U_z[b = k >> 1][n >> 1]; rows past my are zero
@@ -805,7 +823,7 @@ __global__ __launch_bounds__(256) void main_16(const ZMeta* __restrict__ zmeta, 
 #pragma unroll
             for (int it = 0; it < 4; ++it) acc[it] = mfma16<MODE>(ahi[it][s], h[s], acc[it]);
         }
-        tile_out<VEC>(T[yt & 1], acc, oz, ncols, ny, yt, (int64_t)blockIdx.x * 256);
+        tile_out<VEC>(T[yt & 1], acc, oz, ncols, ny, yt, (int64_t)bx * 256);
     };
     vec h0[S], l0[S], h1[S], l1[S];
     yload(yt0, h0, l0);
