@@ -153,8 +153,8 @@ __device__ inline void coef_loop(float (&acc)[8][6], const float2* __restrict__ 
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
-// XCD b % 8), each with its own L2.  The mode contraction (modal_kernel) uses
-// this logical block index instead of blockIdx.x: XCD x takes the contiguous
+// XCD b % 8), each with its own L2.  The mode contraction's short-run class
+// (modal_kernel, MT = 1) uses this logical block index instead of blockIdx.x: XCD x takes the contiguous
 // logical range [x q + min(x, r), ...) (q = n / 8, r = n % 8, a bijection on
 // [0, n)), so the workgroups of one run and mode slot, which read the same
 // phase fragments, share an L2 instead of each XCD fetching them from HBM.
@@ -435,20 +435,23 @@ __global__ __launch_bounds__(256) void main_f32(const ZMeta* __restrict__ zmeta,
                                                 const float* __restrict__ Yfrag, int nx, int ny, int ytiles_per_split,
                                                 float* __restrict__ out, int zoff) {
     constexpr int Q = KMAX / 16;  // 16-byte Y loads per tile
-    const int z = blockIdx.y + zoff;
+    // (plain block order: the XCD-contiguous order of main_16 measured slower
+    // here, 3.62 -> 3.94 ms on the 16-layer stack)
+    const int bx = (int)blockIdx.x, by = (int)blockIdx.y, bz = (int)blockIdx.z;
+    const int z = by + zoff;
     const ZMeta zm = zmeta[z];
     const int K16 = zm.kp >> 4;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t ncols = 6LL * nx;
-    const int64_t n0 = (int64_t)blockIdx.x * 256 + wv * 64;  // waves past ncols compute zeros
+    const int64_t n0 = (int64_t)bx * 256 + wv * 64;  // waves past ncols compute zeros
     const int li = lane & 15, lk = lane >> 4;
     const __amdgpu_buffer_rsrc_t urs = u_slice_rsrc(reinterpret_cast<const float*>(U + zm.u_off), zm.my, ncols);
     const int ytiles = (ny + 15) >> 4;
-    const int yt0 = blockIdx.z * ytiles_per_split;
+    const int yt0 = bz * ytiles_per_split;
     const int yt1 = min(ytiles, yt0 + ytiles_per_split);
     float* oz = out + (int64_t)z * ny * ncols;
     if (K16 == 0) {
-        zero_tiles<VEC>(oz, ncols, ny, yt0, yt1, (int64_t)blockIdx.x * 256);
+        zero_tiles<VEC>(oz, ncols, ny, yt0, yt1, (int64_t)bx * 256);
         return;
     }
     __shared__ float T[2][16][TROW];
@@ -487,7 +490,7 @@ __global__ __launch_bounds__(256) void main_f32(const ZMeta* __restrict__ zmeta,
 #pragma unroll
                     for (int it = 0; it < 4; ++it)
                         acc[it] = __builtin_amdgcn_mfma_f32_16x16x4f32(uf[it][4 * q + e], y[q][e], acc[it], 0, 0, 0);
-        tile_out<VEC>(T[yt & 1], acc, oz, ncols, ny, yt, (int64_t)blockIdx.x * 256);
+        tile_out<VEC>(T[yt & 1], acc, oz, ncols, ny, yt, (int64_t)bx * 256);
     };
     // The first pair of tiles is peeled so that the loop is entered with the
     // same loads and stores in flight as at its back edge.
@@ -1296,7 +1299,10 @@ __global__ __launch_bounds__(256) void modal_kernel(const ModalItem* __restrict_
                                                     const RegionDev* __restrict__ regions,
                                                     const ZMeta* __restrict__ zmeta, const uint32_t* __restrict__ zf,
                                                     int64_t c_elems, float2* __restrict__ C) {
-    const int it = __builtin_amdgcn_readfirstlane(xcd_block() * 4 + (int)(threadIdx.x >> 6));
+    // XCD-contiguous order for the short-run class only: the FP32 64-z class
+    // measured slower with it (64-slab batch, 3.65 -> 4.29 ms)
+    const int lb = MT == 1 ? xcd_block() : (int)blockIdx.x;
+    const int it = __builtin_amdgcn_readfirstlane(lb * 4 + (int)(threadIdx.x >> 6));
     if (it >= nitems) return;
     const ModalItem I = items[it];
     modal_body<MODE, MT>(I, regions[I.region], zmeta, zf, c_elems, C);
