@@ -71,7 +71,10 @@ enum scw_dtype {         /* operand type of the GEMM form's MFMA stage        */
     SCW_DTYPE_FP32 = 0,  /* v_mfma_f32_16x16x4_f32                            */
     SCW_DTYPE_BF16 = 1,  /* v_mfma_f32_16x16x32_bf16                          */
     SCW_DTYPE_BF16X3 = 2,/* split hi+lo bf16: hi*hi + hi*lo + lo*hi           */
-    SCW_DTYPE_FP16 = 3   /* v_mfma_f32_16x16x32_f16                           */
+    SCW_DTYPE_FP16 = 3   /* v_mfma_f32_16x16x32_f16.  Each mode's phase factor  */
+                         /* and amplitude are rescaled by a power of two, but  */
+                         /* the field and its partial sums must stay within    */
+                         /* FP16's range (|value| < 65504).                    */
 };
 
 /* ---- errors / device --------------------------------------------------- */
