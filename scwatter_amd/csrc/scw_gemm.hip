@@ -154,12 +154,13 @@ __device__ inline void coef_loop(float (&acc)[8][6], const float2* __restrict__ 
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
 // XCD b % 8), each with its own L2.  The mode contraction's short-run class
-// (modal_kernel, MT = 1) uses this logical block index instead of blockIdx.x: XCD x takes the contiguous
-// logical range [x q + min(x, r), ...) (q = n / 8, r = n % 8, a bijection on
-// [0, n)), so the workgroups of one run and mode slot, which read the same
-// phase fragments, share an L2 instead of each XCD fetching them from HBM.
-// Tile sample, BF16x3: mode contraction 43.0 -> 38.2 us.  (The VALU item
-// kernel and the 64-z workgroup kernel measured no gain from it.)
+// (modal_kernel, MT = 1) uses this logical block index instead of
+// blockIdx.x: XCD x takes the contiguous logical range [x q + min(x, r), ...)
+// (q = n / 8, r = n % 8, a bijection on [0, n)), so the workgroups of one run
+// and mode slot, which read the same phase fragments, share an L2 instead of
+// each XCD fetching them from HBM.  Tile sample, BF16x3: mode contraction
+// 43.0 -> 38.2 us.  (The VALU item kernel and the 64-z workgroup kernel
+// measured no gain from it; the 16-bit main GEMM uses the 3D form below.)
 __device__ inline int xcd_block() {
     const int nb = (int)gridDim.x, b = (int)blockIdx.x, q = nb >> 3, r = nb & 7, x = b & 7;
     return x * q + min(x, r) + (b >> 3);
