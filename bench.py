@@ -176,8 +176,8 @@ def main():
                     help="only run 5 device-to-device copies of 512 MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the evaluation shards by z-slab with no exchange; see DESIGN.md).", file=sys.stderr)
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported "
+              "(the evaluation would shard by z-slab, with no exchange between GPUs).", file=sys.stderr)
         sys.exit(2)
 
     from scwatter_amd import Exec, Plan, configs, hip

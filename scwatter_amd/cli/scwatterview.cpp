@@ -53,7 +53,7 @@ const char* kHelp =
     "  --input arg (=c.cw)        coupled wave structure (.cw) to evaluate (also positional)\n"
     "  --help                     produce help message\n"
     "  -c [ --cuda ] arg (=0)     device number (-1: the CPU evaluator's rules, still on the GPU)\n"
-    "  --nogui                    save an output file (this build always runs without a GUI)\n"
+    "  --nogui                    save an output file (this program always runs without a GUI)\n"
     "  -v [ --verbose ]           produce verbose output\n"
     "  --size arg (=100)          size of the sampled volume (extent)\n"
     "  --resolution arg (=8)      resolution of the sampled field: N = 2^resolution\n"
