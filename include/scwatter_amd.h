@@ -111,7 +111,8 @@ int scw_exec_create(scw_plan* plan, const double* xs, size_t nx, const double* y
 int scw_exec_run(scw_exec* exec, void* out_device, void* stream);
 /* Launch one stage of a run (profiling hook): 0 = all, 1 = coefficient
  * contraction, 2 = X contraction (16-bit dtypes: the slot sum of the
- * coefficients; their X contraction runs in stage 3), 3 = main MFMA GEMM
+ * coefficients, which also contracts regions of at most 4 terms per cell;
+ * their X contraction runs in stage 3), 3 = main MFMA GEMM
  * (GEMM form); the direct form has a single stage.  Later stages read what
  * earlier stages wrote. */
 int scw_exec_run_stage(scw_exec* exec, int stage, void* out_device, void* stream);

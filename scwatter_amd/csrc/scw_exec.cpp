@@ -337,6 +337,16 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
             ex->nslots_max = std::max(ex->nslots_max, n);
         }
     }
+    // 16-bit dtypes: regions of at most 4 terms per cell (the half-spaces and
+    // layer stacks: one wave per direction and order) get no coefficient
+    // items; the slot-sum kernel sums their terms itself (nslots = 0 marks
+    // them), which saves a launch and a write and read of C_z.
+    std::vector<char> fold(NV, 0);
+    for (int v = 0; v < NV; ++v)
+        if (is16 && !rd[v].dense && rd[v].width > 0 && rd[v].width <= 4 && rd[v].cells > 0) {
+            fold[v] = 1;
+            rd[v].nslots = 0;
+        }
 
     // Per-z metadata, Z rows, C and U offsets.
     std::vector<ZMeta> zm(nz);
@@ -366,7 +376,7 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
             Z.push_back((float)val.real());
             Z.push_back((float)val.imag());
         }
-        f_coef += 24.0 * (double)R.n_terms;
+        (fold[v] ? f_u : f_coef) += 24.0 * (double)R.n_terms;
         f_main += 24.0 * (double)R.my * nx * ny;
         const double cbytes = 24.0 * (double)R.mx * R.my;  // C_z, one slot
         b_coef += cbytes * rd[v].nslots;
@@ -377,10 +387,12 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
             b_u += cbytes * rd[v].nslots + ubytes + 8.0 * R.mx * nx;
             b_main += ubytes;
         } else {
-            // slot sum: slots in, C_z out; the GEMM reads C_z and X and runs
-            // the X contraction of its columns itself
+            // slot sum: slots in (or, folded regions, their terms: 3 float2
+            // and a kz index per term, and the Z row), C_z out; the GEMM reads
+            // C_z and X and runs the X contraction of its columns itself
             f_main += 24.0 * (double)R.mx * R.my * nx;
             b_u += cbytes * rd[v].nslots + cbytes;
+            if (fold[v]) b_u += 28.0 * (double)rd[v].cells * rd[v].width + 8.0 * R.mc;
             b_main += cbytes + 8.0 * R.mx * nx;
         }
     }
@@ -395,7 +407,7 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
         size_t r1 = r0 + 1;  // runs of one group: [r0, r1)
         while (r1 < runs.size() && runs[r1].group == runs[r0].group) ++r1;
         const RegionDev& d = rd[runs[r0].v];
-        if (d.cells == 0 || d.width == 0) {
+        if (d.cells == 0 || d.width == 0 || fold[runs[r0].v]) {
             r0 = r1;
             continue;
         }

@@ -561,13 +561,29 @@ __device__ inline floatx4 mfma16(const typename Op16<MODE>::vec a, const typenam
 __global__ __launch_bounds__(256) void csum_kernel(const ZMeta* __restrict__ zmeta,
                                                    const RegionDev* __restrict__ regions,
                                                    const float2* __restrict__ C, int64_t c_elems,
-                                                   float2* __restrict__ Cs, int zoff) {
+                                                   const float2* __restrict__ Z, float2* __restrict__ Cs, int zoff) {
     const ZMeta zm = zmeta[blockIdx.y + zoff];
     const int ka = regions[zm.region].ka, ns = regions[zm.region].nslots;
     const int e = blockIdx.x * 256 + threadIdx.x;
     if (e >= 3 * zm.my * ka) return;
     const int m = e / ka, a = e - m * ka;
     float2 sum = make_float2(0.0f, 0.0f);
+    if (ns == 0) {
+        // a region of few terms per cell, not contracted before: sum its terms
+        // here, C_z[cell][comp] = sum_j A[j][cell][comp] Z_z[kz index of term j]
+        if (a < zm.mx) {
+            const RegionDev& R = regions[zm.region];
+            const int b = m / 3, comp = m - 3 * b, cell = b * zm.mx + a;
+            for (int j = 0; j < R.width; ++j) {
+                const float2 av = R.ellA[(int64_t)j * R.arow + (int64_t)cell * 3 + comp];
+                const float2 zv = Z[zm.zrow_off + R.ellC[(int64_t)j * R.cells + cell]];
+                sum.x += av.x * zv.x - av.y * zv.y;
+                sum.y += av.x * zv.y + av.y * zv.x;
+            }
+        }
+        Cs[zm.u_off + e] = sum;
+        return;
+    }
     if (a < zm.mx) {
         const int64_t off = zm.c_off + ((int64_t)(m / 3) * zm.mx + a) * 3 + (m % 3);
         for (int k0 = 0; k0 < ns; k0 += 16) {
@@ -1379,7 +1395,7 @@ int launch_gemm(const scw_exec* ex, float* out, hipStream_t st, int stage) {
         else
             hipLaunchKernelGGL(csum_kernel, dim3((unsigned)((3 * ex->my_max * ex->ka_max + 255) / 256), (unsigned)ex->nz),
                                dim3(256), 0, st, ex->d_zmeta.as<ZMeta>(), ex->d_regions.as<RegionDev>(),
-                               ex->d_C.as<float2>(), ex->c_elems, ex->d_U.as<float2>(), 0);
+                               ex->d_C.as<float2>(), ex->c_elems, ex->d_Z.as<float2>(), ex->d_U.as<float2>(), 0);
         SCW_HIP(hipGetLastError());
     }
     if (stage != 0 && stage != 3) return SCW_OK;
