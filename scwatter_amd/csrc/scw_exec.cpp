@@ -397,6 +397,15 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
         }
     }
     ex->c_elems = c_off;
+    // Slot-sum workgroups: 256 elements of one z's C_z each, per z only as
+    // many as its region needs (a batch mixes regions of very different size).
+    std::vector<int2> cblk;
+    if (is16)
+        for (int z = 0; z < nz; ++z) {
+            const int n = 3 * zm[z].my * rd[zm[z].region].ka;
+            for (int e0 = 0; e0 < n; e0 += 256) cblk.push_back(make_int2(z, e0));
+        }
+    ex->ncblk = (int64_t)cblk.size();
 
     // Work items.
     ex->items.clear();
@@ -583,6 +592,7 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
     if ((st = ex->d_mitems.upload(ex->mitems.data(), ex->mitems.size() * sizeof(ModalItem))) != SCW_OK) return st;
     if ((st = ex->d_Z.upload(Z.data(), Z.size() * sizeof(float))) != SCW_OK) return st;
     if ((st = ex->d_Zf.upload(zf.data(), zf.size() * sizeof(uint32_t))) != SCW_OK) return st;
+    if ((st = ex->d_cblk.upload(cblk.data(), cblk.size() * sizeof(int2))) != SCW_OK) return st;
     if ((st = ex->d_X.upload(X.data(), X.size() * sizeof(float))) != SCW_OK) return st;
     if ((st = ex->d_C.alloc((size_t)ex->nslots_max * std::max<int64_t>(c_off, 1) * 8)) != SCW_OK) return st;
     if ((st = ex->d_U.alloc((size_t)std::max<int64_t>(u_off, 1) * 8)) != SCW_OK) return st;
@@ -615,7 +625,7 @@ static void exec_free(scw_exec* ex) {
     if (ex->graph) (void)hipGraphDestroy(ex->graph);
     if (ex->graph_stream) (void)hipStreamDestroy(ex->graph_stream);
     for (DevBuf* b : {&ex->d_dx, &ex->d_dy, &ex->d_z, &ex->d_zslab, &ex->d_items, &ex->d_mitems, &ex->d_regions, &ex->d_zmeta,
-                      &ex->d_Z, &ex->d_Zf, &ex->d_X, &ex->d_C, &ex->d_U, &ex->d_Yfrag, &ex->d_bfpx, &ex->d_out})
+                      &ex->d_Z, &ex->d_Zf, &ex->d_X, &ex->d_C, &ex->d_U, &ex->d_Yfrag, &ex->d_bfpx, &ex->d_cblk, &ex->d_out})
         b->release();
     delete ex;
 }

@@ -561,10 +561,12 @@ __device__ inline floatx4 mfma16(const typename Op16<MODE>::vec a, const typenam
 __global__ __launch_bounds__(256) void csum_kernel(const ZMeta* __restrict__ zmeta,
                                                    const RegionDev* __restrict__ regions,
                                                    const float2* __restrict__ C, int64_t c_elems,
-                                                   const float2* __restrict__ Z, float2* __restrict__ Cs, int zoff) {
-    const ZMeta zm = zmeta[blockIdx.y + zoff];
+                                                   const float2* __restrict__ Z, float2* __restrict__ Cs,
+                                                   const int2* __restrict__ blocks) {
+    const int2 blk = blocks[blockIdx.x];  // (z, first element): only the elements each z has
+    const ZMeta zm = zmeta[blk.x];
     const int ka = regions[zm.region].ka, ns = regions[zm.region].nslots;
-    const int e = blockIdx.x * 256 + threadIdx.x;
+    const int e = blk.y + (int)threadIdx.x;
     if (e >= 3 * zm.my * ka) return;
     const int m = e / ka, a = e - m * ka;
     float2 sum = make_float2(0.0f, 0.0f);
@@ -1392,10 +1394,10 @@ int launch_gemm(const scw_exec* ex, float* out, hipStream_t st, int stage) {
             hipLaunchKernelGGL(u_mfma, g, dim3(256), 0, st, ex->d_zmeta.as<ZMeta>(), ex->d_regions.as<RegionDev>(),
                                ex->d_C.as<float2>(), ex->c_elems, ex->d_X.as<float2>(), (int)ex->nx,
                                ex->d_U.as<float2>(), 0);
-        else
-            hipLaunchKernelGGL(csum_kernel, dim3((unsigned)((3 * ex->my_max * ex->ka_max + 255) / 256), (unsigned)ex->nz),
-                               dim3(256), 0, st, ex->d_zmeta.as<ZMeta>(), ex->d_regions.as<RegionDev>(),
-                               ex->d_C.as<float2>(), ex->c_elems, ex->d_Z.as<float2>(), ex->d_U.as<float2>(), 0);
+        else if (ex->ncblk > 0)
+            hipLaunchKernelGGL(csum_kernel, dim3((unsigned)ex->ncblk), dim3(256), 0, st,
+                               ex->d_zmeta.as<ZMeta>(), ex->d_regions.as<RegionDev>(), ex->d_C.as<float2>(),
+                               ex->c_elems, ex->d_Z.as<float2>(), ex->d_U.as<float2>(), ex->d_cblk.as<int2>());
         SCW_HIP(hipGetLastError());
     }
     if (stage != 0 && stage != 3) return SCW_OK;

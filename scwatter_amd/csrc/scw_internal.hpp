@@ -179,6 +179,8 @@ struct scw_exec {
     int64_t mclass_off[5] = {0, 0, 0, 0, 0};
     scw::DevBuf d_items, d_mitems, d_regions, d_zmeta, d_Z, d_Zf, d_X, d_C, d_U, d_Yfrag;
     scw::DevBuf d_bfpx;       // FP16: shifted exponent planes of the dense regions, then the shifts
+    scw::DevBuf d_cblk;       // 16-bit dtypes: slot-sum workgroups, int2 (z, first element of C_z)
+    int64_t ncblk = 0;
     int64_t tune_coef_items = 0;   // launch-shape choices (0 = default)
     int64_t tune_main_blocks = 0;
     std::vector<double> hx, hy;    // grid kept for re-planning launch shapes
