@@ -250,6 +250,31 @@ def test_direct_plane_matches_volume_slice(c1):
     assert oracle.rel_l2(P, V[:, 5]) < 1e-5  # both sit within FP32 rounding of the oracle
 
 
+@pytest.mark.parametrize("dtype", ["bf16x3", "fp32"])
+def test_half_space_with_uneven_cells(dtype):
+    """A half-space region where a third of the orders lack their reflected
+    wave: cells hold 1 or 2 terms, so the ELL rows are padded.  The 16-bit
+    dtypes contract such narrow regions inside the slot-sum kernel, FP32 in
+    the VALU coefficient kernel; both match the oracle."""
+    from scwatter_amd.cw import WaveTables
+    t = configs.layered_structure([1.0, 1.4 + 0.05j], [0.0], 1.0, 0.5, 9, ("x",)).unpack()
+    b0, e0 = int(t.begin[0]), int(t.end[0])
+    n0 = e0 - b0
+    keep = np.ones(t.W.shape[0], bool)
+    keep[b0 + n0 // 2: e0] = np.arange(n0 - n0 // 2) % 3 != 0  # drop every third wave of the second half
+    W = t.W[keep]
+    shift = np.cumsum(~keep)
+    begin = np.array([b - (shift[b - 1] if b > 0 else 0) for b in t.begin], np.int32)
+    end = np.array([e - shift[e - 1] for e in t.end], np.int32)
+    tt = WaveTables(W, t.z_layers, begin, end)
+    plan = Plan(tt)
+    info = plan.region_info(0)
+    assert info["separable"] and info["width"] == 2 and info["n_terms"] < 2 * info["mx"] * info["my"]
+    xs, ys, zs = configs.axis(-8, 8, 20), configs.axis(-8, 8, 17), configs.axis(-4, 3, 12)
+    ref = oracle.eval_grid_f64(tt.W, tt.z_layers, tt.begin, tt.end, xs, ys, zs)
+    _check(Exec(plan, xs, ys, zs, form="gemm", dtype=dtype).run_host(), ref, TOL[dtype], f"uneven half-space {dtype}")
+
+
 def test_empty_region_gives_zero_field():
     W = np.array([[1, 0, 0, 0.3, 0.1, 1.0]], np.complex128)
     from scwatter_amd.cw import WaveTables
