@@ -16,7 +16,9 @@
 //                 dense eigenmode regions).
 //   X contraction: U_z[b][x][comp] = sum_a C_z[a,b,comp] X[a][x]; FP32:
 //                 u_mfma (MFMA), U_z through HBM; 16-bit dtypes: csum_kernel
-//                 adds the slots, and main_16 contracts X for its own columns.
+//                 adds the slots (and sums the terms of regions of at most 4
+//                 per cell, which get no coefficient items), and main_16
+//                 contracts X for its own columns.
 //   main_*      : E_z[y][x][comp] = sum_b Y[y,b] U_z[b][x][comp] on MFMA.
 //
 // The main GEMM is a real GEMM with the complex product folded into K:
