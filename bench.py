@@ -2,23 +2,30 @@
 """Benchmark: field pts x waves/s and rel-L2 vs the FP64 oracle, direct FP32
 form vs MFMA GEMM form, on one MI355X.
 
-Workload (one step = one evaluation of the whole grid, tables resident in HBM):
-  sCWatter 3D tile sample — 21 x 21 orders (441), 20 sublayers of 1 um between
-  21 boundaries, slab 64 x 64 x 20 um at z in [-10, 10]; 2MF down + 2MF up
-  eigenmodes per sublayer -> 4 MF^2 = 777,924 waves per internal region,
-  15,559,803 waves in all; field on 256 x 256 x 128 points (x, y in [-32, 32],
-  z in [-10, 10]); seeded synthetic eigen-tables.
+Default workload (the largest configuration: 9.99e14 pts x waves per step):
+  batched sweep — 64 sCWatter tile-sample slabs (one 64 x 64 x 4 um slab
+  each, one internal region of 4 MF^2 eigenmode waves), lambda ~ U[0.4, 1.0]
+  um, incidence ~ U[0, 40] deg, orders drawn from {11^2, 21^2, 31^2, 41^2};
+  238M waves in all; each field on 256 x 256 x 64 points; every instance in
+  one GEMM-form run; seeded synthetic eigen-tables.  Launch shapes come from
+  the autotuner's table (scwatter_amd/tuning.json), or are tuned and recorded
+  there when the table has no entry for the run's shape key.
+Other workloads (--workload c1, c2, c3, c5): single interface 64^3, tile
+sample 256^2 x 128, 41^2 x 2-polarisation pupil 512^2 x 256, 16-layer stack
+1024^2 x 512.
 
+A step is one evaluation of the whole grid with the tables resident in HBM.
 The GEMM form is timed over K back-to-back runs, launched eagerly and as
 replays of a captured hipGraph of one run (the faster counts; both are in the
-line); the direct form (seconds per run) is timed over --direct-steps runs.  Device
-time comes from hipEvents recorded on the stream the kernels run on.  The
-CPU baseline is the reference-semantics FP32 port (single thread, like
-cpuEvaluator.cpp) on a bounded sample of the same workload; the same CPU
-leg evaluates the FP64 oracle on seeded sample points, which is what the
-rel-L2 figures are measured against.
+line); the direct form is timed on a subset of the same workload (for the
+batch: one z-slice of the first instance of each order class; its rate is
+linear in the units).  Device time comes from hipEvents recorded on the
+stream the kernels run on.  The CPU leg evaluates the FP64 oracle at seeded
+sample points (what the rel-L2 figures are measured against) and times the
+reference-semantics FP32 port (cpuEvaluator.cpp semantics) on a bounded
+sample, single-threaded like the reference and on all host threads.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--direct-steps D]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c4]
 """
 import argparse
 import re
@@ -92,12 +99,13 @@ def time_stages(ex, out, stream, reps):
     return [t / reps for t in tot]
 
 
-def profiled_traffic(patterns):
+def profiled_traffic(patterns, summary="tile_sample_c2_kernels.md"):
     """HBM bytes per step of the kernels matching `patterns` (regular
-    expressions on the kernel name), from the committed rocprofv3 summary of
-    the tile-sample workload (FETCH_SIZE / WRITE_SIZE per dispatch, scaled
-    against a plain copy); None if the summary lists none of them."""
-    path = os.path.join(ROOT, "profiles", "tile_sample_c2_kernels.md")
+    expressions on the kernel name), from a committed rocprofv3 summary under
+    profiles/ (a table of FETCH_SIZE / WRITE_SIZE per dispatch, scaled against
+    a plain copy, whose last column is MB per step); None if the summary lists
+    none of them."""
+    path = os.path.join(ROOT, "profiles", summary)
     if not os.path.exists(path):
         return None
     total, found = 0.0, False
@@ -154,24 +162,73 @@ WORKLOADS = {
 def sample_device(out, shape, pidx):
     """Copy the field at the sample indices from device memory (24 bytes each)."""
     from scwatter_amd import hip
-    res = np.empty((len(pidx), 3), np.complex64)
-    nz, ny, nx, _ = shape
-    for i, (iz, iy, ix) in enumerate(pidx):
-        off = ((int(iz) * ny + int(iy)) * nx + int(ix)) * 24
-        hip._ok(hip.hip().hipMemcpy(res[i].ctypes.data, out.ptr + off, 24, hip.D2H), "hipMemcpy D2H")
-    return res
+    return hip.gather_points(out.ptr, shape, pidx)
+
+
+def host_info():
+    """Host CPU of this box: model name and logical CPU count."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count()}
+
+
+def cpu_threads():
+    """Threads for the all-core CPU legs: the host's CPUs, at most 16 (one GPU's share of a box)."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def cpu_baselines(tables, xs, ys, z, units_1, desc):
+    """The reference-semantics FP32 port (cpu_cw_evaluate_xy semantics) on
+    whole rows of the plane z: single-threaded on about units_1 pts x waves,
+    and on all host threads on about 10 units_1.  Returns (all-thread record,
+    single-thread record)."""
+    import oracle
+    r = oracle.region_of(float(np.float32(z)), tables.z_layers.astype(np.float32))
+    w_r = int(tables.end[r] - tables.begin[r])
+    out = []
+    for nthr, units in ((cpu_threads(), 10 * units_1), (1, units_1)):
+        # at least one row per thread (the port runs its rows in parallel),
+        # each row cut to as many points as the sample leaves it
+        per_row = xs.size * w_r
+        rows = int(min(ys.size, max(nthr, round(units / max(1, per_row)))))
+        npts = int(min(xs.size, max(1, round(units / (rows * w_r)))))
+        xs32 = xs[:npts].astype(np.float32)
+        ys32 = ys[:rows].astype(np.float32)
+        zs32 = np.array([z], np.float32)
+        n_units = float(npts * rows * w_r)
+        t0 = time.perf_counter()
+        oracle.refcpu_grid(tables.W, tables.z_layers, tables.begin, tables.end, xs32, ys32, zs32, nthreads=nthr)
+        dt = time.perf_counter() - t0
+        out.append({"value": n_units / dt, "unit": UNIT, "cores": nthr, "kind": "port",
+                    "sample": f"cpu_cw_evaluate_xy semantics (FP32, {nthr} thread{'s' if nthr > 1 else ''}) on "
+                              f"{rows} row(s) x {npts} points of the z={z:.3f} plane of {desc} (region {r}, "
+                              f"{w_r} waves) = {n_units:.3e} units in {dt:.1f} s", **host_info()})
+        log(f"[bench] cpu baseline ({nthr} threads): {n_units:.3e} units in {dt:.1f} s -> "
+            f"{n_units / dt:.3e} pts*waves/s")
+    return out[0], out[1]
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--direct-steps", type=int, default=1)
-    ap.add_argument("--sample-points", type=int, default=192)
-    ap.add_argument("--cpu-units", type=float, default=2e8, help="size of the CPU-baseline sample (pts x waves)")
+    ap.add_argument("--sample-points", type=int, default=4096)
+    ap.add_argument("--cpu-units", type=float, default=2e8,
+                    help="size of the single-thread CPU-baseline sample (pts x waves); the all-thread sample is "
+                         "10x larger")
     ap.add_argument("--dtypes", default="fp32,bf16x3,bf16,fp16")
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c4", choices=sorted(WORKLOADS))
+    ap.add_argument("--retune", action="store_true", help="time the launch-shape candidates even if the "
+                    "tuning table has an entry for the run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device-to-device copies of 512 MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -267,19 +324,9 @@ def main():
         f["rel_l2"] = oracle.rel_l2(f["sample"], ref)
         f["max_abs"] = oracle.max_abs(f["sample"], ref)
         log(f"[bench] {k}: rel-L2 {f['rel_l2']:.3e} max-abs {f['max_abs']:.3e}")
-    # CPU baseline: cpu_cw_evaluate_xy semantics on whole rows of the slice nearest z = 0.3 um,
-    # as many rows as make about --cpu-units units.
+    # CPU baseline: cpu_cw_evaluate_xy semantics on whole rows of the slice nearest z = 0.3 um
     iz = int(np.argmin(np.abs(zs - 0.3)))
-    r = oracle.region_of(float(np.float32(zs[iz])), tables.z_layers.astype(np.float32))
-    w_r = int(tables.end[r] - tables.begin[r])
-    rows = int(max(1, min(ys.size, round(args.cpu_units / max(1, xs.size * w_r)))))
-    xs32, ys32, zs32 = xs.astype(np.float32), ys[:rows].astype(np.float32), zs[iz:iz + 1].astype(np.float32)
-    cpu_units = float(xs.size * rows * w_r)
-    t0 = time.perf_counter()
-    oracle.refcpu_grid(tables.W, tables.z_layers, tables.begin, tables.end, xs32, ys32, zs32, nthreads=1)
-    cpu_s = time.perf_counter() - t0
-    cpu_rate = cpu_units / cpu_s
-    log(f"[bench] cpu baseline: {cpu_units:.3e} units in {cpu_s:.1f} s -> {cpu_rate:.3e} pts*waves/s")
+    cpu_all, cpu_one = cpu_baselines(tables, xs, ys, float(zs[iz]), args.cpu_units, args.workload)
 
     # ---- headline: fastest GEMM dtype within the accuracy bar ----
     ok = [k for k in forms if k.startswith("gemm_") and forms[k]["rel_l2"] <= ACCURACY_BAR]
@@ -313,31 +360,43 @@ def main():
                       "max_abs": f["max_abs"], **({"stages_ms": {s: v["ms"] for s, v in f["stages"].items()}}
                                                   if "stages" in f else {})} for k, f in forms.items()},
         "roofline": roof, "roofline_direct": roof_direct,
-        "cpu_baseline": {"value": cpu_rate, "unit": UNIT, "cores": 1, "kind": "port",
-                         "sample": f"cpu_cw_evaluate_xy semantics (FP32, 1 thread) on {rows} row(s) x "
-                                   f"{xs.size} points of the z={zs[iz]:.3f} slice (region {r}, {w_r} waves) = "
-                                   f"{cpu_units:.3e} units"},
+        "cpu_baseline": cpu_all, "cpu_baseline_single_thread": cpu_one,
     }
     print(json.dumps(line), flush=True)
+
+
+# Oracle sample points per instance of the batch, by order count per axis:
+# 21 + 15 + 12 + 16 instances of 11^2 / 21^2 / 31^2 / 41^2 orders take 4,116
+# points in all, fewer on the larger instances (a point's oracle sum runs over
+# all 4 MF^2 waves of its region: 58,564 ... 11,303,044).
+BATCH_POINTS = {11: 100, 21: 64, 31: 48, 41: 30}
 
 
 def bench_batch(args):
     """Batched sweep (config 4): every instance in one GEMM-form run, launch shape from the autotuner."""
     import oracle
-    from scwatter_amd import BatchExec, Plan, autotune, configs, hip
+    from scwatter_amd import BatchExec, Exec, Plan, autotune, configs, hip
 
     t0 = time.perf_counter()
     params = configs.config4_instances()
     xs, ys, zs = configs.config4_grid()
     nzi = zs.size
     rng = np.random.default_rng(20261203)
-    samples = [(i, int(rng.integers(nzi)), int(rng.integers(ys.size)), int(rng.integers(xs.size)))
-               for i in range(len(params)) for _ in range(3)]
-    # Each instance is planned, checked by the oracle at its 3 sample points
-    # (the CPU leg's checker), and its host tables dropped, except the
-    # largest instance's, kept for the CPU baseline.
-    plans, ref_parts, waves, big = [], [], 0, None
-    nthr = min(16, os.cpu_count() or 1)
+    scale = args.sample_points / 4096.0
+    samples = []  # (instance, iz, iy, ix)
+    for i, p in enumerate(params):
+        for _ in range(max(1, int(round(BATCH_POINTS.get(p["M"], 64) * scale)))):
+            samples.append((i, int(rng.integers(nzi)), int(rng.integers(ys.size)), int(rng.integers(xs.size))))
+    # the direct form's subset: the middle z-slice of the first instance of each order class
+    first = {}
+    for i, p in enumerate(params):
+        first.setdefault(p["M"], i)
+    direct_set = sorted(first.values())
+    # Each instance is planned, checked by the oracle at its sample points
+    # (the CPU leg's checker), and its host tables dropped, except those of
+    # the direct subset, kept for the direct form's checks and the CPU baseline.
+    plans, ref_parts, waves, keep = [], [], 0, {}
+    nthr = cpu_threads()
     t_or = 0.0
     for i, p in enumerate(params):
         t = configs.config4_structure(p).unpack()
@@ -347,22 +406,29 @@ def bench_batch(args):
         pts = [[xs[x], ys[y], zs[z]] for j, z, y, x in samples if j == i]
         ref_parts.append(oracle.eval_points_f64(t.W, t.z_layers, t.begin, t.end, pts, nthreads=nthr))
         t_or += time.perf_counter() - t2
-        if big is None or t.W.shape[0] > big.W.shape[0]:
-            big = t
+        if i in direct_set:
+            keep[i] = t
         del t
         if i % 8 == 7:
-            log(f"[bench] c4: {i + 1} instances planned ({time.perf_counter()-t0:.1f} s)")
+            log(f"[bench] c4: {i + 1} instances planned ({time.perf_counter()-t0:.1f} s, oracle {t_or:.1f} s)")
     ref = np.concatenate(ref_parts)
     log(f"[bench] c4: {len(plans)} instances, {waves} waves, orders "
-        f"{sorted(collections.Counter(p['M'] for p in params).items())} ({time.perf_counter()-t0:.1f} s, "
-        f"oracle {t_or:.1f} s)")
+        f"{sorted(collections.Counter(p['M'] for p in params).items())}, {len(samples)} sample points "
+        f"({time.perf_counter()-t0:.1f} s, oracle {t_or:.1f} s on {nthr} threads)")
     stream = hip.Stream()
     forms = {}
     for dt in [d for d in args.dtypes.split(",") if d in ("bf16x3", "fp32", "bf16", "fp16")]:
         t1 = time.perf_counter()
         bx = BatchExec(plans, xs, ys, [zs] * len(plans), dtype=dt)
         out = hip.DeviceArray(bx.shape, np.complex64)
-        best, cand = autotune.tune(bx, out.ptr, stream, reps=3)
+        key = autotune.shape_key(bx, "c4")
+        cached = None if args.retune else autotune.apply_cached(bx, key)
+        if cached:
+            tuned = {"coef_items": cached["coef_items"], "main_blocks": cached["main_blocks"], "source": "table"}
+        else:
+            best, cand = autotune.tune(bx, out.ptr, stream, reps=3, key=key, path=autotune.TABLE)
+            tuned = {"coef_items": best[0], "main_blocks": best[1], "source": "tuned",
+                     "candidates_ms": {f"{a}/{b}": round(v, 4) for (a, b), v in cand.items()}}
         info = bx.info()
         ms, wall_ms, way, by_way = time_steps(bx, out, stream, args.steps, args.warmup)
         sms = time_stages(bx, out, stream, 3)
@@ -373,37 +439,75 @@ def bench_batch(args):
         pidx = np.array([(i * nzi + z, y, x) for i, z, y, x in samples])
         samp = sample_device(out, bx.shape, pidx)
         forms[f"gemm_{dt}"] = {"ms": ms, "wall_ms": wall_ms, "launch": way, "ms_by_launch": by_way,
-                               "rate": info["units"] / (ms * 1e-3), "info": info,
-                               "stages": stages, "sample": samp, "tuned": {"coef_items": best[0], "main_blocks": best[1],
-                                                                           "candidates_ms": {f"{a}/{b}": round(v, 4)
-                                                                                             for (a, b), v in cand.items()}}}
-        log(f"[bench] c4 gemm {dt}: {ms:.3f} ms/step, {info['units'] / (ms * 1e-3):.3e} pts*waves/s, tuned {best}, "
-            "stages " + ", ".join(f"{k} {v['ms']:.3f} ms" for k, v in stages.items()) +
-            f" ({time.perf_counter()-t1:.1f} s)")
+                               "rate": info["units"] / (ms * 1e-3), "info": info, "stages": stages, "sample": samp,
+                               "tuned": tuned}
+        log(f"[bench] c4 gemm {dt}: {ms:.3f} ms/step ({way}; {by_way}), {info['units'] / (ms * 1e-3):.3e} "
+            f"pts*waves/s, launch shape {tuned}, stages " +
+            ", ".join(f"{k} {v['ms']:.3f} ms" for k, v in stages.items()) + f" ({time.perf_counter()-t1:.1f} s)")
         bx.destroy()
         out.free()
     for k, f in forms.items():
         f["rel_l2"] = oracle.rel_l2(f["sample"], ref)
         f["max_abs"] = oracle.max_abs(f["sample"], ref)
         log(f"[bench] c4 {k}: rel-L2 {f['rel_l2']:.3e} max-abs {f['max_abs']:.3e}")
-    # CPU baseline: reference-semantics FP32 port on a bounded sample of the largest instance
-    tb = big
-    w_r = int(tb.end[1] - tb.begin[1])
-    npts = int(max(1, min(xs.size, round(args.cpu_units / w_r))))
+
+    # ---- direct form on the subset (rate linear in units) ----
     t1 = time.perf_counter()
-    oracle.refcpu_grid(tb.W, tb.z_layers, tb.begin, tb.end, xs[:npts].astype(np.float32),
-                       ys[:1].astype(np.float32), zs[nzi // 2:nzi // 2 + 1].astype(np.float32), nthreads=1)
-    cpu_s = time.perf_counter() - t1
-    cpu_units = float(npts * w_r)
-    ok = [k for k in forms if forms[k]["rel_l2"] <= ACCURACY_BAR]
-    best = max(ok or list(forms), key=lambda k: forms[k]["rate"])
+    iz = nzi // 2
+    d_units, d_ms, d_samp, d_ref = 0.0, 0.0, [], []
+    e0, e1 = hip.Event(), hip.Event()
+    warm = Exec(plans[direct_set[0]], xs[:8], ys[:8], zs[:1], form="direct")
+    warm.run_host()
+    warm.destroy()
+    for i in direct_set:
+        ex = Exec(plans[i], xs, ys, zs[iz:iz + 1], form="direct")
+        out = hip.DeviceArray(ex.shape, np.complex64)
+        hip.synchronize()
+        e0.record(stream)
+        ex.run(out.ptr, stream.handle)
+        e1.record(stream)
+        e1.synchronize()
+        d_ms += e0.elapsed_ms(e1)
+        d_units += ex.info()["units"]
+        t = keep[i]
+        didx = np.stack([np.zeros(16, int), rng.integers(0, ys.size, 16), rng.integers(0, xs.size, 16)], 1)
+        d_samp.append(sample_device(out, ex.shape, didx))
+        pts = np.stack([xs[didx[:, 2]], ys[didx[:, 1]], np.full(16, zs[iz])], 1)
+        d_ref.append(oracle.eval_points_f64(t.W, t.z_layers, t.begin, t.end, pts, nthreads=nthr))
+        ex.destroy()
+        out.free()
+    d_rate = d_units / (d_ms * 1e-3)
+    d_samp, d_ref = np.concatenate(d_samp), np.concatenate(d_ref)
+    forms["direct_fp32"] = {"ms": d_ms, "rate": d_rate, "rel_l2": oracle.rel_l2(d_samp, d_ref),
+                            "max_abs": oracle.max_abs(d_samp, d_ref), "units": d_units,
+                            "subset": f"z-slice {iz} of instances {direct_set} (orders "
+                                      f"{[params[i]['M'] for i in direct_set]}), 256 x 256 points each"}
+    log(f"[bench] c4 direct fp32 on the subset: {d_units:.3e} units in {d_ms:.1f} ms -> {d_rate:.3e} "
+        f"pts*waves/s, rel-L2 {forms['direct_fp32']['rel_l2']:.3e} ({time.perf_counter()-t1:.1f} s)")
+
+    # ---- CPU baselines on the largest instance of the direct subset ----
+    big = max(direct_set, key=lambda i: params[i]["M"])
+    cpu_all, cpu_one = cpu_baselines(keep[big], xs, ys, float(zs[iz]), args.cpu_units,
+                                     f"instance {big} ({params[big]['M']}^2 orders)")
+
+    gemm = [k for k in forms if k.startswith("gemm_")]
+    ok = [k for k in gemm if forms[k]["rel_l2"] <= ACCURACY_BAR]
+    best = max(ok or gemm, key=lambda k: forms[k]["rate"])
     B = forms[best]
     dt = best[len("gemm_"):]
     dom = max(B["stages"], key=lambda k: B["stages"][k]["ms"])
     S = B["stages"][dom]
-    roof = roofline(S["flops"], S["bytes"], S["ms"], PEAK["fp32"] if dt == "fp32" else PEAK["bf16"])
-    roof["kernel"] = " + ".join(stage_kernels(dom, dt))
+    peak = PEAK["fp32"] if dt == "fp32" or dom != "main" else PEAK["bf16"]
+    roof = roofline(S["flops"], S["bytes"], S["ms"], peak)
+    kpat = stage_kernels(dom, dt)
+    roof["kernel"] = " + ".join(kpat)
     roof["stage"] = dom
+    roof["traffic"] = profiled_traffic(kpat, "batched_sweep_c4_kernels.md")
+    D = forms["direct_fp32"]
+    D_flops = 36.0 * D["units"]
+    roof_direct = {"bound": "valu", "achieved": round(D_flops / (D["ms"] * 1e-3) / 1e12, 2), "peak": PEAK["fp32"],
+                   "unit": "TFLOP/s", "frac": round(D_flops / (D["ms"] * 1e-3) / 1e12 / PEAK["fp32"], 4),
+                   "note": "36 algorithmic FP32 flops + 3 transcendentals per (point, wave), on the direct subset"}
     line = {
         "metric": METRIC, "value": B["rate"], "unit": UNIT, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": B["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dt,
@@ -411,15 +515,14 @@ def bench_batch(args):
         "config": {"workload": WORKLOADS["c4"][0], "units_per_step": B["info"]["units"], "form": "gemm",
                    "gemm_dtype": dt, "instances": len(plans), "waves": waves},
         "rel_l2": B["rel_l2"], "max_abs": B["max_abs"], "accuracy_sample_points": len(samples),
+        "gemm_over_direct": B["rate"] / D["rate"],
         "forms": {k: {"ms_per_step": round(f["ms"], 4), **({"launch": f["launch"], "ms_by_launch": f["ms_by_launch"]}
                                                           if "launch" in f else {}),
-                      "rate": f["rate"], "rel_l2": f["rel_l2"],
-                      "max_abs": f["max_abs"], "stages_ms": {s: v["ms"] for s, v in f["stages"].items()},
-                      "tuned": f["tuned"]} for k, f in forms.items()},
-        "roofline": roof,
-        "cpu_baseline": {"value": cpu_units / cpu_s, "unit": UNIT, "cores": 1, "kind": "port",
-                         "sample": f"cpu_cw_evaluate_xy semantics (FP32, 1 thread) on {npts} points of one row of the "
-                                   f"largest instance ({w_r} waves) = {cpu_units:.3e} units"},
+                      "rate": f["rate"], "rel_l2": f["rel_l2"], "max_abs": f["max_abs"],
+                      **({"stages_ms": {s2: v["ms"] for s2, v in f["stages"].items()}, "tuned": f["tuned"]}
+                         if "stages" in f else {"subset": f["subset"]})} for k, f in forms.items()},
+        "roofline": roof, "roofline_direct": roof_direct,
+        "cpu_baseline": cpu_all, "cpu_baseline_single_thread": cpu_one,
     }
     print(json.dumps(line), flush=True)
 

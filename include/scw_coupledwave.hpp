@@ -88,12 +88,15 @@ class CoupledWaveStructure {
         return true;
     }
 
-    // Region index of z: 1 + the last boundary with z >= Layers[l].z.
+    // CoupledWaveStructure.h:141-147, kept as the reference has it: l + 1 for
+    // the FIRST boundary l with z >= Layers[l].z, else 0.  With ascending
+    // boundaries that is 1 for every z at or above the first boundary, which
+    // is not the evaluator's region rule (1 + the LAST such l, scwatter_amd.h);
+    // the reference never calls it, and neither does this library.
     size_t getPlaneIndex(T z) const {
-        size_t r = 0;
         for (size_t l = 0; l < Layers.size(); l++)
-            if (z >= Layers[l].z) r = l + 1;
-        return r;
+            if (z >= Layers[l].z) return l + 1;
+        return 0;
     }
 
    private:

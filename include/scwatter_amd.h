@@ -103,7 +103,11 @@ int scw_plan_device_bytes(const scw_plan* plan, size_t* bytes);
 
 /* ---- grid execution (replaces gpu_cw_evaluate / kernel_{xy,xz,yz}) ------ */
 /* Rectilinear grid: xs[nx], ys[ny], zs[nz] (host, double).  Builds the
- * per-grid tables (FP64 on the host, rounded once) and allocates scratch. */
+ * per-grid tables (FP64 on the host, rounded once) and allocates scratch.
+ * With SCW_FORM_AUTO, a separable structure whose grid the GEMM form cannot
+ * take (a phase factor exp(i kz (z - zref)) past FP32's range, far from the
+ * region's phase origin in a strongly evanescent mode) gets the direct form;
+ * scw_exec_info reports the form chosen. */
 int scw_exec_create(scw_plan* plan, const double* xs, size_t nx, const double* ys, size_t ny,
                     const double* zs, size_t nz, double region_tol, int form, int dtype, scw_exec** exec);
 /* Launch the evaluation into out_device (nz*ny*nx*3 complex64) on `stream`
@@ -139,15 +143,19 @@ int scw_batch_exec_create(int n, scw_plan* const* plans, const double* xs, size_
                           const double* const* zs, const size_t* nz, double region_tol, int dtype, scw_exec** exec);
 /* Launch shape of the GEMM form: the target number of coefficient work
  * items (wave-sized) and of main-GEMM workgroups; 0 keeps the default.
- * Rebuilds the work lists (drops a captured graph).  Used by the autotuner. */
+ * Rebuilds the work lists (drops a captured graph).  Used by the autotuner.
+ * If the rebuild fails (e.g. SCW_ERR_NOMEM), every run, stage, graph capture
+ * and graph launch of the exec returns SCW_ERR_ARG until a later call
+ * succeeds; the exec can always be destroyed. */
 int scw_exec_set_launch(scw_exec* exec, int64_t coef_items, int64_t main_blocks);
 /* Capture one run into a hipGraph and replay it with scw_exec_graph_launch. */
 int scw_exec_graph_capture(scw_exec* exec, void* out_device, void* stream);
 int scw_exec_graph_launch(scw_exec* exec, void* stream);
 
 /* ---- On-device post-processing of fields (sCWatterView.cpp:194-352) -----
- * Device buffers, asynchronous on `stream`, no allocation.  Results are
- * bit-identical to the reference's float arithmetic.
+ * Device buffers, asynchronous on `stream`, no allocation.  Scalars and
+ * colours are bit-identical to the reference's float arithmetic; min/max
+ * follow its comparisons with two stated exceptions (scw_post_minmax).
  * scw_post_scalar: EvaluateScalarSlices (sCWatterView.cpp:299-352).  field:
  *   complex64 [n][3]; scalar: complex64 [n] = E[mode] (0 x, 1 y, 2 z) or
  *   mode 3: sum_c E_c conj(E_c) (the intensity, imaginary part 0). */
@@ -155,8 +163,11 @@ int scw_post_scalar(const void* field_device, size_t n, int mode, void* scalar_d
 /* scw_post_minmax: CalculateMinMax (sCWatterView.cpp:264-296).  minmax: float
  * [4] on the device = (min Re, max Re, min Im, max Im) of scalar[0..n); with
  * accumulate != 0 the previous contents are included, so several slices give
- * one range.  NaN entries are skipped (the reference skips them except at
- * index 0); no finite entry gives NaN. */
+ * one range.  As in the reference, the seed is scalar[0] (or, with
+ * accumulate, the previous contents): a NaN seed makes that result NaN, a NaN
+ * anywhere else is skipped, and no entry at all gives NaN.  Exceptions: a NaN
+ * result is the quiet NaN 0x7fc00000 (not the seed's bits), and -0.0 counts
+ * as below +0.0 (the reference keeps whichever of the two it met first). */
 int scw_post_minmax(const void* scalar_device, size_t n, float* minmax_device, int accumulate, void* stream);
 /* scw_post_color: EvaluateColorValue / EvaluateColorSlices
  * (sCWatterView.cpp:194-262).  lowhigh: float[4] on the device = (real_low,
@@ -167,6 +178,12 @@ int scw_post_minmax(const void* scalar_device, size_t n, float* minmax_device, i
  * undefined there). */
 int scw_post_color(const void* scalar_device, size_t n, const float* lowhigh_device, const float real_color[3],
                    const float imag_color[3], int show_real, int show_imag, void* rgb_device, void* stream);
+
+/* ---- test hook (no counterpart in the reference) -------------------------
+ * Makes the (k+1)-th device allocation from now on fail as SCW_ERR_NOMEM
+ * (k < 0: off), so that tests can drive the allocation-failure paths without
+ * exhausting the device.  Process-wide; not for production use. */
+void scw_test_fail_alloc_after(int64_t k);
 
 /* ---- .cw files (CoupledWaveStructure.h:7-134) --------------------------- */
 int scw_cw_load(const char* path, scw_cw** cw);

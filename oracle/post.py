@@ -29,15 +29,47 @@ def scalar(E, mode):
 
 def minmax(slices):
     """CalculateMinMax (sCWatterView.cpp:264-296) over a list of complex64 arrays:
-    (min Re, max Re, min Im, max Im), NaN entries skipped (NaN if none is left)."""
+    (min Re, max Re, min Im, max Im).  The four values are seeded from
+    slices[0][0] and updated with strict < and > (sCWatterView.cpp:271-289), so
+    a NaN seed stays NaN and every other NaN is skipped; NaN if there are no
+    entries.  A NaN result is the quiet NaN; among -0.0 and +0.0 the minimum is
+    -0.0 (the reference keeps the first it met; those ties are left out of the
+    tests)."""
     re = np.concatenate([np.asarray(s, np.complex64).ravel().real for s in slices]).astype(F)
     im = np.concatenate([np.asarray(s, np.complex64).ravel().imag for s in slices]).astype(F)
 
     def mm(v):
+        if v.size == 0 or np.isnan(v[0]):
+            return F(np.nan), F(np.nan)
         v = v[~np.isnan(v)]
-        return (F(np.nan), F(np.nan)) if v.size == 0 else (v.min(), v.max())
+        return v.min(), v.max()
     (a, b), (c, d) = mm(re), mm(im)
     return np.array([a, b, c, d], F)
+
+
+def minmax_loop(slices):
+    """The same as minmax, as the reference's loop itself: seed from
+    slices[0][0], then for every index i the slices in order, each compared
+    with > and <.  Pure Python; for small inputs (it pins minmax)."""
+    first = np.asarray(slices[0], np.complex64).ravel()
+    if first.size == 0:
+        return np.array([np.nan] * 4, F)
+    rmax = rmin = F(first[0].real)
+    imax = imin = F(first[0].imag)
+    n = first.size
+    flat = [np.asarray(s, np.complex64).ravel() for s in slices]
+    for i in range(n):
+        for s in flat:
+            r, m = F(s[i].real), F(s[i].imag)
+            if r > rmax:
+                rmax = r
+            if r < rmin:
+                rmin = r
+            if m > imax:
+                imax = m
+            if m < imin:
+                imin = m
+    return np.array([rmin, rmax, imin, imax], F)
 
 
 def _clamp(v):  # sCWatterView.cpp:194-199

@@ -30,6 +30,7 @@
 //
 // Parity status of each function is stated in tests/test_oracle_pins.py.
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstddef>
@@ -60,43 +61,79 @@ struct Wave64 {
     bool zero;
 };
 
+void prepare_one(const double* r, Wave64& o) {
+    cd e0[3] = {cd(r[0], r[1]), cd(r[2], r[3]), cd(r[4], r[5])};
+    double s = std::max(std::abs(e0[0]), std::max(std::abs(e0[1]), std::abs(e0[2])));
+    o.zero = !(s > 0.0);
+    o.lns = o.zero ? 0.0 : std::log(s);
+    for (int c = 0; c < 3; ++c) o.e[c] = o.zero ? cd(0, 0) : e0[c] / s;
+    o.k[0] = cd(r[6], r[7]);
+    o.k[1] = cd(r[8], r[9]);
+    o.k[2] = cd(r[10], r[11]);
+}
+
 std::vector<Wave64> prepare(const double* w, int64_t n) {
     std::vector<Wave64> out((size_t)n);
-    for (int64_t i = 0; i < n; ++i) {
-        const double* r = w + 12 * i;
-        Wave64& o = out[(size_t)i];
-        cd e0[3] = {cd(r[0], r[1]), cd(r[2], r[3]), cd(r[4], r[5])};
-        double s = std::max(std::abs(e0[0]), std::max(std::abs(e0[1]), std::abs(e0[2])));
-        o.zero = !(s > 0.0);
-        o.lns = o.zero ? 0.0 : std::log(s);
-        for (int c = 0; c < 3; ++c) o.e[c] = o.zero ? cd(0, 0) : e0[c] / s;
-        o.k[0] = cd(r[6], r[7]);
-        o.k[1] = cd(r[8], r[9]);
-        o.k[2] = cd(r[10], r[11]);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) prepare_one(w + 12 * i, out[(size_t)i]);
+    return out;
+}
+
+// Only the waves of the regions flagged in `used` (the others stay unset).
+std::vector<Wave64> prepare_regions(const double* w, int64_t n, const int32_t* begin, const int32_t* end, int layers,
+                                    const std::vector<char>& used) {
+    std::vector<Wave64> out((size_t)n);
+    for (int r = 0; r <= layers; ++r) {
+        if (!used[(size_t)r]) continue;
+#pragma omp parallel for schedule(static)
+        for (int64_t i = begin[r]; i < end[r]; ++i) prepare_one(w + 12 * i, out[(size_t)i]);
     }
     return out;
 }
 
 // Sum over the waves [b, e) of a region at one point (cpuEvaluator.cpp:322-329).
+// Complex products are written out in real arithmetic (std::complex's
+// operator* calls the C99 NaN-recovering multiply, a library call per
+// product); for finite operands the values are the same textbook products.
+inline void term(const Wave64& w, double x, double y, double z, double* E) {
+    // k . r = (kx x + ky y) + kz z, real and imaginary parts
+    const double krr = (w.k[0].real() * x + w.k[1].real() * y) + w.k[2].real() * z;
+    const double kri = (w.k[0].imag() * x + w.k[1].imag() * y) + w.k[2].imag() * z;
+    // exp(i kr + ln s) = exp(-Im kr + ln s) * (cos Re kr + i sin Re kr)
+    const double mag = std::exp(w.lns - kri);
+    double sn, cs;
+    sincos(krr, &sn, &cs);
+    const double pr = mag * cs, pi = mag * sn;
+    for (int c = 0; c < 3; ++c) {
+        const double er = w.e[c].real(), ei = w.e[c].imag();
+        E[2 * c] += er * pr - ei * pi;
+        E[2 * c + 1] += er * pi + ei * pr;
+    }
+}
+
 inline void sum_point(const std::vector<Wave64>& W, int64_t b, int64_t e, double x, double y, double z,
                       double* out6) {
-    cd E[3] = {cd(0, 0), cd(0, 0), cd(0, 0)};
+    double E[6] = {0, 0, 0, 0, 0, 0};
+    for (int64_t i = b; i < e; ++i) {
+        const Wave64& w = W[(size_t)i];
+        if (!w.zero) term(w, x, y, z, E);
+    }
+    for (int c = 0; c < 6; ++c) out6[c] = E[c];
+}
+
+// The same sum for a block of up to PB points of one region, each wave read
+// once for the whole block (the tables of large regions do not fit in cache).
+constexpr int PB = 16;
+inline void sum_block(const std::vector<Wave64>& W, int64_t b, int64_t e, const double* const* p, int np,
+                      double* const* out6) {
+    double E[PB][6] = {};
     for (int64_t i = b; i < e; ++i) {
         const Wave64& w = W[(size_t)i];
         if (w.zero) continue;
-        cd kr = w.k[0] * x + w.k[1] * y + w.k[2] * z;
-        // exp(i kr + ln s) = exp(-Im kr + ln s) * (cos Re kr + i sin Re kr)
-        double mag = std::exp(w.lns - kr.imag());
-        double ph = kr.real();
-        cd p(mag * std::cos(ph), mag * std::sin(ph));
-        E[0] += w.e[0] * p;
-        E[1] += w.e[1] * p;
-        E[2] += w.e[2] * p;
+        for (int q = 0; q < np; ++q) term(w, p[q][0], p[q][1], p[q][2], E[q]);
     }
-    for (int c = 0; c < 3; ++c) {
-        out6[2 * c] = E[c].real();
-        out6[2 * c + 1] = E[c].imag();
-    }
+    for (int q = 0; q < np; ++q)
+        for (int c = 0; c < 6; ++c) out6[q][c] = E[q][c];
 }
 
 #ifdef _OPENMP
@@ -122,8 +159,8 @@ void oracle_eval_grid_f64(const double* waves, int64_t n_waves, const double* z_
                           const int32_t* begin, const int32_t* end, const double* xs, int64_t nx,
                           const double* ys, int64_t ny, const double* zs, int64_t nz, double tol,
                           double* out, int nthreads) {
-    std::vector<Wave64> W = prepare(waves, n_waves);
     set_threads(nthreads);
+    std::vector<Wave64> W = prepare(waves, n_waves);
     const int64_t rows = nz * ny;
 #pragma omp parallel for schedule(dynamic, 1)
     for (int64_t row = 0; row < rows; ++row) {
@@ -135,16 +172,32 @@ void oracle_eval_grid_f64(const double* waves, int64_t n_waves, const double* z_
 }
 
 // FP64 field at explicit points pts[n][3] (x, y, z); out double[n][3][2].
+// Points are grouped by region and taken PB at a time.
 void oracle_eval_points_f64(const double* waves, int64_t n_waves, const double* z_layers, int layers,
                             const int32_t* begin, const int32_t* end, const double* pts, int64_t n,
                             double tol, double* out, int nthreads) {
-    std::vector<Wave64> W = prepare(waves, n_waves);
     set_threads(nthreads);
-#pragma omp parallel for schedule(dynamic, 4)
-    for (int64_t i = 0; i < n; ++i) {
-        const double x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
-        const int r = region_of(z, z_layers, layers, tol);
-        sum_point(W, begin[r], end[r], x, y, z, out + 6 * i);
+    std::vector<std::vector<int64_t>> by_region((size_t)layers + 1);
+    for (int64_t i = 0; i < n; ++i) by_region[(size_t)region_of(pts[3 * i + 2], z_layers, layers, tol)].push_back(i);
+    std::vector<char> used((size_t)layers + 1);
+    for (int r = 0; r <= layers; ++r) used[(size_t)r] = !by_region[(size_t)r].empty();
+    std::vector<Wave64> W = prepare_regions(waves, n_waves, begin, end, layers, used);
+    std::vector<std::pair<int, int64_t>> blocks;  // (region, first index into by_region[region])
+    for (int r = 0; r <= layers; ++r)
+        for (size_t k = 0; k < by_region[(size_t)r].size(); k += PB) blocks.push_back({r, (int64_t)k});
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t bi = 0; bi < (int64_t)blocks.size(); ++bi) {
+        const int r = blocks[(size_t)bi].first;
+        const std::vector<int64_t>& idx = by_region[(size_t)r];
+        const int64_t k0 = blocks[(size_t)bi].second;
+        const int np = (int)std::min<int64_t>(PB, (int64_t)idx.size() - k0);
+        const double* p[PB];
+        double* o[PB];
+        for (int q = 0; q < np; ++q) {
+            p[q] = pts + 3 * idx[(size_t)(k0 + q)];
+            o[q] = out + 6 * idx[(size_t)(k0 + q)];
+        }
+        sum_block(W, begin[r], end[r], p, np, o);
     }
 }
 

@@ -620,6 +620,20 @@ static std::vector<ZEntry> entries(const scw_exec* ex) {
     return ze;
 }
 
+// Drops the GEMM-form tables and work lists (before a fall back to the direct form).
+static void release_gemm(scw_exec* ex) {
+    for (DevBuf* b : {&ex->d_items, &ex->d_mitems, &ex->d_regions, &ex->d_zmeta, &ex->d_Z, &ex->d_Zf, &ex->d_X, &ex->d_C,
+                      &ex->d_U, &ex->d_Yfrag, &ex->d_bfpx, &ex->d_cblk})
+        b->release();
+    ex->items.clear();
+    ex->mitems.clear();
+    for (int k = 0; k <= 4; ++k) ex->mclass_off[k] = 0;
+    ex->ncblk = 0;
+    ex->my_max = ex->ka_max = ex->kmax = 0;
+    ex->nslots_max = 1;
+    for (int k = 0; k < 4; ++k) ex->stage_flops[k] = ex->stage_bytes[k] = 0.0;
+}
+
 static void exec_free(scw_exec* ex) {
     if (ex->graph_exec) (void)hipGraphExecDestroy(ex->graph_exec);
     if (ex->graph) (void)hipGraphDestroy(ex->graph);
@@ -678,6 +692,7 @@ int scw_exec_create(scw_plan* plan, const double* xs, size_t nx, const double* y
         }
     }
     ex->units = units;
+    const bool auto_form = form == SCW_FORM_AUTO;
     if (form == SCW_FORM_AUTO) form = all_sep ? SCW_FORM_GEMM : SCW_FORM_DIRECT;
     if (form == SCW_FORM_GEMM && !all_sep) {
         const int st = fail(SCW_ERR_UNSUPPORTED, "GEMM form: region %d is not separable (%s)", first_bad,
@@ -700,6 +715,16 @@ int scw_exec_create(scw_plan* plan, const double* xs, size_t nx, const double* y
         ex->hz.assign(zs, zs + nz);
         ex->hzp.assign(nz, 0);
         st = form == SCW_FORM_DIRECT ? prep_direct(ex, xs, ys, zs) : prep_gemm(ex, xs, ys, entries(ex));
+        if (st == SCW_ERR_UNSUPPORTED && auto_form && form == SCW_FORM_GEMM) {
+            // Separable, but this grid takes a phase table past FP32's range
+            // (a point far from the region's phase origin in a strongly
+            // evanescent mode) or a U slice past 32-bit offsets: AUTO falls
+            // back to the direct form, which carries such exponents in FP64.
+            release_gemm(ex);
+            ex->form = SCW_FORM_DIRECT;
+            ex->dtype = SCW_DTYPE_FP32;
+            st = prep_direct(ex, xs, ys, zs);
+        }
     } catch (const std::bad_alloc&) {
         st = fail(SCW_ERR_NOMEM, "host allocation failed while building grid tables");
     }
@@ -798,16 +823,29 @@ int scw_exec_set_launch(scw_exec* ex, int64_t coef_items, int64_t main_blocks) {
     ex->tune_coef_items = coef_items;
     ex->tune_main_blocks = main_blocks;
     ex->nslots_max = 1;
+    // prep_gemm replaces the device tables one by one; a failure part way
+    // leaves some of them released next to rebuilt work lists, so the exec
+    // stays refused until a later rebuild succeeds.
+    ex->invalid = true;
+    int st;
     try {
-        return prep_gemm(ex, ex->hx.data(), ex->hy.data(), entries(ex));
+        st = prep_gemm(ex, ex->hx.data(), ex->hy.data(), entries(ex));
     } catch (const std::bad_alloc&) {
-        return fail(SCW_ERR_NOMEM, "host allocation failed while building grid tables");
+        st = fail(SCW_ERR_NOMEM, "host allocation failed while building grid tables");
     }
+    if (st == SCW_OK) ex->invalid = false;
+    return st;
+}
+
+static int refuse_invalid(const scw_exec* ex) {
+    return fail(SCW_ERR_ARG, "exec is unusable: its last scw_exec_set_launch failed (%s)",
+                ex->form == SCW_FORM_GEMM ? "rebuild the launch shape or destroy it" : "destroy it");
 }
 
 int scw_exec_run(scw_exec* ex, void* out_device, void* stream) {
     if (!ex) return fail(SCW_ERR_ARG, "exec is NULL");
     if (!out_device) return fail(SCW_ERR_ARG, "output pointer is NULL");
+    if (ex->invalid) return refuse_invalid(ex);
     DeviceGuard g(ex->plan->device);
     if (g.st != SCW_OK) return g.st;
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -819,6 +857,7 @@ int scw_exec_run_stage(scw_exec* ex, int stage, void* out_device, void* stream) 
     if (!ex) return fail(SCW_ERR_ARG, "exec is NULL");
     if (!out_device) return fail(SCW_ERR_ARG, "output pointer is NULL");
     if (stage < 0 || stage > 3) return fail(SCW_ERR_ARG, "stage %d out of range", stage);
+    if (ex->invalid) return refuse_invalid(ex);
     DeviceGuard g(ex->plan->device);
     if (g.st != SCW_OK) return g.st;
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -829,6 +868,7 @@ int scw_exec_run_stage(scw_exec* ex, int stage, void* out_device, void* stream) 
 int scw_exec_run_host(scw_exec* ex, void* out_host, void* stream) {
     if (!ex) return fail(SCW_ERR_ARG, "exec is NULL");
     if (!out_host) return fail(SCW_ERR_ARG, "output pointer is NULL");
+    if (ex->invalid) return refuse_invalid(ex);
     DeviceGuard g(ex->plan->device);
     if (g.st != SCW_OK) return g.st;
     const size_t bytes = (size_t)ex->nx * ex->ny * ex->nz * 24;
@@ -874,6 +914,7 @@ int scw_exec_stage_info(const scw_exec* ex, int stage, double* flops, double* by
 int scw_exec_graph_capture(scw_exec* ex, void* out_device, void* stream) {
     if (!ex) return fail(SCW_ERR_ARG, "exec is NULL");
     if (!out_device) return fail(SCW_ERR_ARG, "output pointer is NULL");
+    if (ex->invalid) return refuse_invalid(ex);
     DeviceGuard g(ex->plan->device);
     if (g.st != SCW_OK) return g.st;
     if (ex->graph_exec) {
@@ -906,6 +947,7 @@ int scw_exec_graph_capture(scw_exec* ex, void* out_device, void* stream) {
 
 int scw_exec_graph_launch(scw_exec* ex, void* stream) {
     if (!ex || !ex->graph_exec) return fail(SCW_ERR_ARG, "no captured graph");
+    if (ex->invalid) return refuse_invalid(ex);
     DeviceGuard g(ex->plan->device);
     if (g.st != SCW_OK) return g.st;
     SCW_HIP(hipGraphLaunch(ex->graph_exec, static_cast<hipStream_t>(stream)));

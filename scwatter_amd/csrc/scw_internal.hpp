@@ -161,6 +161,10 @@ struct scw_exec {
     double units = 0, flops = 0, bytes = 0;
     double stage_flops[4] = {0, 0, 0, 0}, stage_bytes[4] = {0, 0, 0, 0};
     int launches = 0;
+    // Set while the tables are being rebuilt (scw_exec_set_launch) and left
+    // set when a rebuild fails: the device tables may then be missing or
+    // inconsistent with the work lists, so every launch entry point refuses.
+    bool invalid = false;
     std::vector<int> zregion;
     // direct form
     double xc = 0, yc = 0;

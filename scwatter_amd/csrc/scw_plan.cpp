@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <array>
+#include <atomic>
 #include <unordered_map>
 #include <new>
 
@@ -42,9 +43,16 @@ int hip_fail(hipError_t e, const char* what) {
                 hipGetErrorString(e));
 }
 
+// Allocation-failure injection for tests of the error paths: after
+// scw_test_fail_alloc_after(k), the (k+1)-th device allocation from then on
+// fails as out-of-memory (k < 0 turns it off).
+static std::atomic<int64_t> g_fail_after{-1};
+
 int DevBuf::alloc(size_t n) {
     release();
     if (n == 0) return SCW_OK;
+    if (g_fail_after.load() >= 0 && g_fail_after.fetch_sub(1) == 0)
+        return fail(SCW_ERR_NOMEM, "hipMalloc: injected allocation failure (%zu bytes)", n);
     hipError_t e = hipMalloc(&p, n);
     if (e != hipSuccess) {
         p = nullptr;
@@ -275,6 +283,7 @@ using namespace scw;
 extern "C" {
 
 const char* scw_last_error(void) { return g_err; }
+void scw_test_fail_alloc_after(int64_t k) { g_fail_after.store(k < 0 ? -1 : k); }
 int scw_abi_version(void) { return SCW_ABI_VERSION; }
 
 int scw_device_count(int* count) {

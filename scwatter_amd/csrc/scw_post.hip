@@ -51,13 +51,23 @@ __global__ __launch_bounds__(256) void post_scalar_kernel(const float2* __restri
 }
 
 // mm holds 4 keys (min Re, max Re, min Im, max Im) while reducing; floats
-// before and after.
-__global__ void post_mm_prepare(uint32_t* mm, int accumulate) {
+// before and after.  CalculateMinMax seeds its four values from S_xy[0] and
+// then compares with < and >, so a NaN seed never changes again (every
+// comparison with it is false) while a NaN met later is skipped.  Here the
+// seed is scalar[0] (or, with accumulate, the previous contents), and a NaN
+// seed is kept as the key no data value can reach: 0 for a minimum slot,
+// 0xffffffff for a maximum slot (keys of NaNs only, which the reduction skips).
+__global__ void post_mm_prepare(uint32_t* mm, const float2* __restrict__ S, int64_t n, int accumulate) {
     const int t = threadIdx.x;
     if (t >= 4) return;
-    const uint32_t init = (t & 1) ? 0u : 0xffffffffu;
-    const float prev = __uint_as_float(mm[t]);
-    mm[t] = (accumulate && !isnan(prev)) ? fkey(prev) : init;  // NaN: nothing seen yet
+    const bool is_max = (t & 1) != 0;
+    const uint32_t init = is_max ? 0u : 0xffffffffu, sticky = is_max ? 0xffffffffu : 0u;
+    uint32_t k = init;
+    if (accumulate || n > 0) {
+        const float seed = accumulate ? __uint_as_float(mm[t]) : (t < 2 ? S[0].x : S[0].y);
+        k = isnan(seed) ? sticky : fkey(seed);
+    }
+    mm[t] = k;
 }
 
 __global__ __launch_bounds__(256) void post_mm_reduce(const float2* __restrict__ S, int64_t n, uint32_t* mm) {
@@ -92,9 +102,12 @@ __global__ __launch_bounds__(256) void post_mm_reduce(const float2* __restrict__
     }
 }
 
+// Keys back to floats; a NaN seed, or no entry at all, gives the quiet NaN.
 __global__ void post_mm_finish(uint32_t* mm) {
     const int t = threadIdx.x;
-    if (t < 4) mm[t] = __float_as_uint(fkey_inv(mm[t]));
+    if (t >= 4) return;
+    const float f = fkey_inv(mm[t]);
+    mm[t] = isnan(f) ? 0x7fc00000u : __float_as_uint(f);
 }
 
 __device__ inline float clamp01(float v) {  // sCWatterView.cpp:194-199
@@ -155,7 +168,8 @@ int scw_post_minmax(const void* scalar_device, size_t n, float* minmax_device, i
     if (!minmax_device || (n > 0 && !scalar_device)) return fail(SCW_ERR_ARG, "NULL buffer");
     hipStream_t st = static_cast<hipStream_t>(stream);
     uint32_t* mm = reinterpret_cast<uint32_t*>(minmax_device);
-    hipLaunchKernelGGL(post_mm_prepare, dim3(1), dim3(64), 0, st, mm, accumulate ? 1 : 0);
+    hipLaunchKernelGGL(post_mm_prepare, dim3(1), dim3(64), 0, st, mm, static_cast<const float2*>(scalar_device),
+                       (int64_t)n, accumulate ? 1 : 0);
     if (n > 0)
         hipLaunchKernelGGL(post_mm_reduce, post_grid((int64_t)n), dim3(256), 0, st,
                            static_cast<const float2*>(scalar_device), (int64_t)n, mm);

@@ -136,3 +136,18 @@ class Event:
         if self.handle:
             hip().hipEventDestroy(self.handle)
             self.handle = None
+
+
+def gather_points(ptr, shape, idx):
+    """Field values at grid indices idx[n] = (iz, iy, ix) of a complex64
+    (nz, ny, nx, 3) field in device memory at ptr: (n, 3) complex64, one
+    24-byte copy per point (for accuracy samples of fields too large to copy)."""
+    nz, ny, nx, _ = shape
+    res = np.empty((len(idx), 3), np.complex64)
+    h = hip()
+    for i, (iz, iy, ix) in enumerate(idx):
+        if not (0 <= iz < nz and 0 <= iy < ny and 0 <= ix < nx):
+            raise IndexError(f"point {i}: index {(iz, iy, ix)} outside {shape[:3]}")
+        off = ((int(iz) * ny + int(iy)) * nx + int(ix)) * 24
+        _ok(h.hipMemcpy(res[i].ctypes.data, ptr + off, 24, D2H), "hipMemcpy D2H")
+    return res

@@ -405,3 +405,76 @@ def test_batch_of_uneven_instances(dtype, runs):
     s.synchronize()
     assert np.array_equal(buf.to_host(), E2)
     s.destroy()
+
+
+@pytest.mark.parametrize("dtype", ["bf16x3", "bf16", "fp16"])
+def test_pupil_35_staged_x_contraction(dtype):
+    """35 x 35 orders: 35 distinct kx and ky, so the 16-bit main GEMM runs
+    with K = 96 and stages its X fragments in LDS over three K steps (ka =
+    48), the path of every 33..48-order lattice; full grid of 14,400 points
+    against the oracle."""
+    t = configs.layered_structure([1.0, 1.4 + 0.05j], [0.0], 1.0, 0.5, 35, ("x",)).unpack()
+    plan = Plan(t)
+    for r in (0, 1):
+        info = plan.region_info(r)
+        assert info["separable"] and 33 <= info["mx"] <= 48 and 33 <= info["my"] <= 48
+    xs, ys, zs = configs.axis(-15, 15, 40), configs.axis(-14, 16, 36), configs.axis(-6, 4, 10)
+    ref = oracle.eval_grid_f64(t.W, t.z_layers, t.begin, t.end, xs, ys, zs)
+    _check(Exec(plan, xs, ys, zs, form="gemm", dtype=dtype).run_host(), ref, {"bf16x3": 1e-4, "bf16": 1e-2,
+                                                                             "fp16": 2e-3}[dtype],
+           f"35x35 pupil gemm {dtype}")
+
+
+def test_failed_relaunch_refuses_runs(tile_small):
+    """A failed rebuild of the launch shape (an allocation failing part way
+    through, forced with the library's allocation-failure hook) leaves the
+    exec refusing every launch entry point, without launching anything; a
+    later successful rebuild makes it usable again."""
+    from scwatter_amd import ScwError, _native, hip
+    plan, g, ref = tile_small
+    ex = Exec(plan, *g, form="gemm", dtype="bf16x3")
+    E0 = ex.run_host()
+    L = _native.lib()
+    L.scw_test_fail_alloc_after(2)  # the third allocation of the rebuild fails
+    try:
+        with pytest.raises(ScwError, match="NOMEM"):
+            ex.set_launch(coef_items=1024, main_blocks=256)
+    finally:
+        L.scw_test_fail_alloc_after(-1)
+    buf = hip.DeviceArray(ex.shape, np.complex64)
+    s = hip.Stream()
+    with pytest.raises(ScwError, match="ARG"):
+        ex.run(buf.ptr)
+    with pytest.raises(ScwError, match="ARG"):
+        ex.run_stage(3, buf.ptr)
+    with pytest.raises(ScwError, match="ARG"):
+        ex.run_host()
+    with pytest.raises(ScwError, match="ARG"):
+        ex.graph_capture(buf.ptr, s.handle)
+    ex.set_launch(coef_items=1024, main_blocks=256)
+    E1 = ex.run_host()
+    assert oracle.rel_l2(E1, E0) < 1e-6
+    _check(E1, ref, TOL["bf16x3"], "tile sample after a failed and a good rebuild")
+    s.destroy()
+
+
+def test_auto_falls_back_to_direct_when_phase_overflows():
+    """A 30 um slab at 0.4 um with evanescent modes: the region is separable,
+    but exp(i kz (z - zref)) reaches e^90 at the slab faces, past FP32's
+    range, so the GEMM form refuses this grid and AUTO takes the direct form
+    instead of failing."""
+    from scwatter_amd import ScwError
+    cw = configs.tile_sample_structure(M=3, Z=1, size=(6.0, 6.0, 30.0), lam=0.4, seed=21,
+                                       evanescent_fraction=0.5, incidence_deg=5.0)
+    t = cw.unpack()
+    plan = Plan(t)
+    assert plan.region_info(1)["separable"]
+    kz = t.W[t.begin[1]:t.end[1], 5]
+    xs, ys, zs = configs.axis(-3, 3, 12), configs.axis(-3, 3, 10), configs.axis(-14.9, 14.9, 24)
+    assert np.abs(kz.imag).max() * 14.9 > np.log(1e30)
+    with pytest.raises(ScwError, match="UNSUPPORTED"):
+        Exec(plan, xs, ys, zs, form="gemm", dtype="bf16x3")
+    ex = Exec(plan, xs, ys, zs, dtype="bf16x3")
+    assert ex.info()["form"] == "direct"
+    ref = oracle.eval_grid_f64(t.W, t.z_layers, t.begin, t.end, xs, ys, zs)
+    _check(ex.run_host(), ref, 1e-5, "thick evanescent slab, AUTO")

@@ -12,21 +12,25 @@ from scwatter_amd import Exec, Plan, configs, hip, post
 pytestmark = pytest.mark.gpu
 
 
-def _field(n, seed):
+def _field(n, seed, nan0=0):
     rng = np.random.default_rng(seed)
     E = (rng.standard_normal((n, 3)) * np.exp(rng.uniform(-20, 20, (n, 3)))
          + 1j * rng.standard_normal((n, 3))).astype(np.complex64)
-    E.real[::97, 0] = np.nan
+    E.real[nan0::97, 0] = np.nan
     E.imag[::89, 1] = np.inf
     E.real[::83, 2] = -0.0
     E[min(5, n - 1)] = 0
     return E
 
 
+@pytest.mark.parametrize("nan0", [0, 1])
 @pytest.mark.parametrize("n", [1, 1000, 100003])
-def test_scalar_minmax_color_bit_exact(n):
+def test_scalar_minmax_color_bit_exact(n, nan0):
+    """nan0 = 0: the x component's real part is NaN at index 0 (the
+    reference's seed), so its range is NaN for the whole call sequence; nan0 =
+    1: the NaN entries start at index 1 and are skipped."""
     s = hip.Stream()
-    slices = [_field(n, 7 + k) for k in range(3)]
+    slices = [_field(n, 7 + k, nan0) for k in range(3)]
     dS = [hip.DeviceArray((n,), np.complex64) for _ in range(3)]
     mm = hip.DeviceArray((4,), np.float32)
     rgb = hip.DeviceArray((n, 3), np.uint8)
