@@ -378,6 +378,9 @@ def bench_batch(args):
     from scwatter_amd import BatchExec, Exec, Plan, autotune, configs, hip
 
     t0 = time.perf_counter()
+    from scwatter_amd import lib
+    lib()
+    oracle._load()  # both loaded before the thread pool uses them
     params = configs.config4_instances()
     xs, ys, zs = configs.config4_grid()
     nzi = zs.size
@@ -392,29 +395,31 @@ def bench_batch(args):
     for i, p in enumerate(params):
         first.setdefault(p["M"], i)
     direct_set = sorted(first.values())
-    # Each instance is planned, checked by the oracle at its sample points
-    # (the CPU leg's checker), and its host tables dropped, except those of
-    # the direct subset, kept for the direct form's checks and the CPU baseline.
-    plans, ref_parts, waves, keep = [], [], 0, {}
+    # Each instance is generated, planned and checked by the oracle at its
+    # sample points (the CPU leg's checker) on a pool of 8 threads, and its
+    # host tables dropped, except those of the direct subset, kept for the
+    # direct form's checks and the CPU baseline.
     nthr = cpu_threads()
-    t_or = 0.0
-    for i, p in enumerate(params):
-        t = configs.config4_structure(p).unpack()
-        plans.append(Plan(t, device=0))
-        waves += t.W.shape[0]
+    pts_of = collections.defaultdict(list)
+    for j, z, y, x in samples:
+        pts_of[j].append([xs[x], ys[y], zs[z]])
+
+    def build(i, t):
+        plan = Plan(t, device=0)
         t2 = time.perf_counter()
-        pts = [[xs[x], ys[y], zs[z]] for j, z, y, x in samples if j == i]
-        ref_parts.append(oracle.eval_points_f64(t.W, t.z_layers, t.begin, t.end, pts, nthreads=nthr))
-        t_or += time.perf_counter() - t2
-        if i in direct_set:
-            keep[i] = t
-        del t
-        if i % 8 == 7:
-            log(f"[bench] c4: {i + 1} instances planned ({time.perf_counter()-t0:.1f} s, oracle {t_or:.1f} s)")
+        r = oracle.eval_points_f64(t.W, t.z_layers, t.begin, t.end, pts_of[i], nthreads=max(1, nthr // 8))
+        return plan, r, t.W.shape[0], time.perf_counter() - t2, (t if i in direct_set else None)
+    built = configs.map_instances(build, params, workers=8)
+    plans = [b[0] for b in built]
+    ref_parts = [b[1] for b in built]
+    waves = sum(b[2] for b in built)
+    t_or = sum(b[3] for b in built)
+    keep = {i: b[4] for i, b in enumerate(built) if b[4] is not None}
+    del built
     ref = np.concatenate(ref_parts)
     log(f"[bench] c4: {len(plans)} instances, {waves} waves, orders "
         f"{sorted(collections.Counter(p['M'] for p in params).items())}, {len(samples)} sample points "
-        f"({time.perf_counter()-t0:.1f} s, oracle {t_or:.1f} s on {nthr} threads)")
+        f"({time.perf_counter()-t0:.1f} s; oracle {t_or:.1f} thread-pool s)")
     stream = hip.Stream()
     forms = {}
     for dt in [d for d in args.dtypes.split(",") if d in ("bf16x3", "fp32", "bf16", "fp16")]:
@@ -426,7 +431,7 @@ def bench_batch(args):
         if cached:
             tuned = {"coef_items": cached["coef_items"], "main_blocks": cached["main_blocks"], "source": "table"}
         else:
-            best, cand = autotune.tune(bx, out.ptr, stream, reps=3, key=key, path=autotune.TABLE)
+            best, cand = autotune.tune(bx, out.ptr, stream, reps=4, key=key, path=autotune.TABLE)
             tuned = {"coef_items": best[0], "main_blocks": best[1], "source": "tuned",
                      "candidates_ms": {f"{a}/{b}": round(v, 4) for (a, b), v in cand.items()}}
         info = bx.info()

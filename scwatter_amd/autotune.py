@@ -36,23 +36,31 @@ def apply_cached(ex, key, path=TABLE):
     return e
 
 
-def tune(ex, out_ptr, stream, reps=5, candidates=CANDIDATES, key=None, path=None):
-    """Time each candidate (reps full runs after one warm-up), keep the fastest.
+def tune(ex, out_ptr, stream, reps=5, candidates=CANDIDATES, key=None, path=None, rounds=2):
+    """Time each candidate and keep the fastest.
 
-    Returns (best (coef_items, main_blocks), {candidate: ms}).  With `path`,
-    the result is merged into that JSON table under `key`.
+    Every candidate is timed `rounds` times, the rounds interleaved over the
+    candidates (so that a drift of the clock or of the caches does not favour
+    one of them), each time over `reps` back-to-back runs after two warm-up
+    runs (the first run after a rebuild reads freshly written tables); a
+    candidate's time is its best round.  Returns (best (coef_items,
+    main_blocks), {candidate: ms}).  With `path`, the result is merged into
+    that JSON table under `key`.
     """
     e0, e1 = hip.Event(), hip.Event()
     times = {}
-    for ci, mb in candidates:
-        ex.set_launch(ci, mb)
-        ex.run(out_ptr, stream.handle)
-        e0.record(stream)
-        for _ in range(reps):
-            ex.run(out_ptr, stream.handle)
-        e1.record(stream)
-        e1.synchronize()
-        times[(ci, mb)] = e0.elapsed_ms(e1) / reps
+    for _ in range(rounds):
+        for ci, mb in candidates:
+            ex.set_launch(ci, mb)
+            for _ in range(2):
+                ex.run(out_ptr, stream.handle)
+            e0.record(stream)
+            for _ in range(reps):
+                ex.run(out_ptr, stream.handle)
+            e1.record(stream)
+            e1.synchronize()
+            ms = e0.elapsed_ms(e1) / reps
+            times[(ci, mb)] = min(times.get((ci, mb), ms), ms)
     best = min(times, key=times.get)
     ex.set_launch(*best)
     if path:

@@ -296,3 +296,19 @@ def config4_grid(nx=256, ny=256, nz=64, size=(64.0, 64.0, 4.0)):
     h = size[2] / 2
     return (axis(-size[0] / 2, size[0] / 2, nx), axis(-size[1] / 2, size[1] / 2, ny),
             axis(-h + 0.05, h - 0.05, nz))
+
+
+def map_instances(fn, params, workers=8):
+    """fn(i, config4_structure(params[i]).unpack()) for every instance, on a
+    pool of threads (numpy's random fills and copies and the library's plan
+    builder run without the interpreter lock); results in instance order.
+    Each instance's tables are generated from its own seed, so the results do
+    not depend on the number of workers."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(i):
+        return fn(i, config4_structure(params[i]).unpack())
+    if workers <= 1:
+        return [one(i) for i in range(len(params))]
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        return list(pool.map(one, range(len(params))))

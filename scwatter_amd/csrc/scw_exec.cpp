@@ -354,6 +354,7 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
     int64_t u_off = 0, c_off = 0;
     double f_coef = 0, f_u = 0, f_main = 0;
     double b_coef = 0, b_u = 0, b_main = 24.0 * (double)nx * ny * nz;
+    std::vector<char> x_counted(NV, 0);  // X_r is needed once per run, whatever the number of its z
     for (int z = 0; z < nz; ++z) {
         const int v = zv[z];
         const RegionPlan& R = vreg[v].first->regions[vreg[v].second];
@@ -384,8 +385,9 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
             // X contraction kernel: slots and X in, U_z out; the GEMM reads U_z
             const double ubytes = 24.0 * (double)R.my * nx;
             f_u += 24.0 * (double)R.mx * R.my * nx;
-            b_u += cbytes * rd[v].nslots + ubytes + 8.0 * R.mx * nx;
+            b_u += cbytes * rd[v].nslots + ubytes + (x_counted[v] ? 0.0 : 8.0 * R.mx * nx);
             b_main += ubytes;
+            x_counted[v] = 1;
         } else {
             // slot sum: slots in (or, folded regions, their terms: 3 float2
             // and a kz index per term, and the Z row), C_z out; the GEMM reads
@@ -393,7 +395,8 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
             f_main += 24.0 * (double)R.mx * R.my * nx;
             b_u += cbytes * rd[v].nslots + cbytes;
             if (fold[v]) b_u += 28.0 * (double)rd[v].cells * rd[v].width + 8.0 * R.mc;
-            b_main += cbytes + 8.0 * R.mx * nx;
+            b_main += cbytes + (x_counted[v] ? 0.0 : 8.0 * R.mx * nx);
+            x_counted[v] = 1;
         }
     }
     ex->c_elems = c_off;
