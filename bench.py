@@ -227,6 +227,7 @@ def main():
                          "10x larger")
     ap.add_argument("--dtypes", default="fp32,bf16x3,bf16,fp16")
     ap.add_argument("--workload", default="c4", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-drop-in", action="store_true", help="leave out the drop-in latency measurement")
     ap.add_argument("--retune", action="store_true", help="time the launch-shape candidates even if the "
                     "tuning table has an entry for the run")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -363,6 +364,59 @@ def main():
         "cpu_baseline": cpu_all, "cpu_baseline_single_thread": cpu_one,
     }
     print(json.dumps(line), flush=True)
+
+
+def drop_in_latency(N=256, reps=10):
+    """End-to-end latency of the reference's per-event call, gpu_cw_evaluate
+    (gpuEvaluator.cu:131-165, called on every GUI event by
+    sCWatterView.cpp:354-376): the three N x N planes through scw_eval_planes
+    into device buffers, then copied to host arrays (the reference's three
+    synchronous copies), wall time per call on the host clock.  Views of 20 um
+    centred at the origin (d = 20 / (N - 1), the viewer's rule), on the tile
+    sample and the 16-layer stack: the first call (execs built), the same
+    call repeated (cached execs), and a drag of the xy plane by one grid step
+    per call (one plane rebuilt per call), for AUTO (the GEMM form) and the
+    direct form."""
+    from scwatter_amd import Plan, configs, eval_planes, hip
+    out = {}
+    for name, gen in (("c2", configs.config2), ("c5", configs.config5)):
+        t0 = time.perf_counter()
+        cw, _ = gen()
+        plan = Plan(cw.unpack(), device=0)
+        del cw
+        extent = 20.0
+        d = np.float32(extent / (N - 1))
+        start = np.float32(-extent / 2)
+        bufs = [hip.DeviceArray((N, N, 3), np.complex64) for _ in range(3)]
+        host = [np.empty((N, N, 3), np.complex64) for _ in range(3)]
+        H = hip.hip()
+
+        def call(form, zpos):
+            eval_planes(plan, start, start, start, 0.1, -0.1, zpos, d, N, *(b.ptr for b in bufs), form=form)
+            for hb, db in zip(host, bufs):
+                hip._ok(H.hipMemcpy(hb.ctypes.data, db.ptr, hb.nbytes, hip.D2H), "hipMemcpy D2H")
+        res = {}
+        for form in ("auto", "direct"):
+            t1 = time.perf_counter()
+            call(form, 0.35)
+            first = time.perf_counter() - t1
+            t1 = time.perf_counter()
+            for _ in range(reps):
+                call(form, 0.35)
+            repeat = (time.perf_counter() - t1) / reps
+            t1 = time.perf_counter()
+            for k in range(reps):
+                call(form, 0.35 + float(d) * (k + 1))
+            move = (time.perf_counter() - t1) / reps
+            res[form] = {"first_ms": round(first * 1e3, 3), "repeat_ms": round(repeat * 1e3, 3),
+                         "move_one_plane_ms": round(move * 1e3, 3)}
+        for b in bufs:
+            b.free()
+        plan.destroy()
+        out[name] = res
+        log(f"[bench] drop-in gpu_cw_evaluate {name} N={N}: {res} ({time.perf_counter() - t0:.1f} s)")
+    return {"N": N, "extent_um": 20.0, "what": "scw_eval_planes + 3 device-to-host copies, wall ms per call",
+            **out}
 
 
 # Oracle sample points per instance of the batch, by order count per axis:
@@ -528,6 +582,7 @@ def bench_batch(args):
                          if "stages" in f else {"subset": f["subset"]})} for k, f in forms.items()},
         "roofline": roof, "roofline_direct": roof_direct,
         "cpu_baseline": cpu_all, "cpu_baseline_single_thread": cpu_one,
+        "drop_in_latency": None if args.no_drop_in else drop_in_latency(),
     }
     print(json.dumps(line), flush=True)
 

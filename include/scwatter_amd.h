@@ -152,6 +152,23 @@ int scw_exec_set_launch(scw_exec* exec, int64_t coef_items, int64_t main_blocks)
 int scw_exec_graph_capture(scw_exec* exec, void* out_device, void* stream);
 int scw_exec_graph_launch(scw_exec* exec, void* stream);
 
+/* ---- the reference's three planes (gpu_cw_evaluate, gpuEvaluator.cu:131-165) ---
+ * Layouts as gpu_cw_evaluate: E_xy[iy N + ix] (plane
+ * z), E_xz[iz N + ix] (plane y), E_yz[iz N + iy] (plane x), each N x N x 3
+ * complex64 in device memory (NULL skips that plane), with the axis
+ * coordinates start + d * i rounded as one fused multiply-add
+ * (gpuEvaluator.cu:78-79) and the region rule z >= z_l - region_tol (0.001 is
+ * the reference GPU rule, gpuEvaluator.cu:34).  form / dtype as for
+ * scw_exec_create.  Asynchronous on `stream`.  The plan keeps the execs of
+ * the last six planes (keyed by plane, N, d, starts, fixed coordinate,
+ * region_tol, form and dtype): a repeated plane launches at once, a changed
+ * one is rebuilt (host tables and device uploads, into the buffers of the
+ * least recently used exec once six are held).  Calls on one plan must not
+ * overlap. */
+int scw_eval_planes(scw_plan* plan, float x_start, float y_start, float z_start, float x, float y, float z,
+                    float d, int N, double region_tol, int form, int dtype, void* E_xy, void* E_xz, void* E_yz,
+                    void* stream);
+
 /* ---- On-device post-processing of fields (sCWatterView.cpp:194-352) -----
  * Device buffers, asynchronous on `stream`, no allocation.  Scalars and
  * colours are bit-identical to the reference's float arithmetic; min/max
@@ -180,7 +197,8 @@ int scw_post_color(const void* scalar_device, size_t n, const float* lowhigh_dev
                    const float imag_color[3], int show_real, int show_imag, void* rgb_device, void* stream);
 
 /* ---- test hook (no counterpart in the reference) -------------------------
- * Makes the (k+1)-th device allocation from now on fail as SCW_ERR_NOMEM
+ * Makes the (k+1)-th device buffer request from now on (a new allocation or
+ * the reuse of a large enough one) fail as SCW_ERR_NOMEM
  * (k < 0: off), so that tests can drive the allocation-failure paths without
  * exhausting the device.  Process-wide; not for production use. */
 void scw_test_fail_alloc_after(int64_t k);

@@ -27,6 +27,8 @@ namespace {
 
 std::vector<scw_planewave64> g_w64;  // FP64 copy of W (what cw_unpack narrowed)
 scw_plan* g_plan = nullptr;
+scw::DevBuf g_frame;  // gpu_cw_evaluate's three device planes, kept between calls
+int g_frame_dev = -1;  // the device g_frame lives on
 int g_status = SCW_OK;
 int g_form = SCW_FORM_AUTO;
 int g_dtype = SCW_DTYPE_FP32;
@@ -64,13 +66,6 @@ std::vector<double> cpu_axis(float start, float d, size_t N) {
         volatile float t = (float)(unsigned)i * d;
         v[i] = (double)(float)(start + t);
     }
-    return v;
-}
-
-// gpuEvaluator.cu:78-79: start + d * i, contracted to one fused multiply-add.
-std::vector<double> gpu_axis(float start, float d, size_t N) {
-    std::vector<double> v(N);
-    for (size_t i = 0; i < N; ++i) v[i] = (double)std::fmaf(d, (float)(unsigned)i, start);
     return v;
 }
 
@@ -165,11 +160,32 @@ void gpu_cw_evaluate(std::complex<float>* E_xy, std::complex<float>* E_xz, std::
         g_status = scw::fail(SCW_ERR_NOMEM, "insufficient GPU memory for an %dx%d frame", N, N);
         return;
     }
-    const double tol = 0.001;
-    const std::vector<double> xs = gpu_axis(x_start, d, n), ys = gpu_axis(y_start, d, n), zs = gpu_axis(z_start, d, n);
-    if ((g_status = eval_plane(E_xy, xs, ys, {(double)z}, tol)) != SCW_OK) return;
-    if ((g_status = eval_plane(E_xz, xs, {(double)y}, zs, tol)) != SCW_OK) return;
-    g_status = eval_plane(E_yz, {(double)x}, ys, zs, tol);
+    // The three planes through scw_eval_planes (its per-plan cache keeps the
+    // execs of unchanged planes) into one device frame kept between calls,
+    // then copied to the caller's host arrays as the reference does
+    // (gpuEvaluator.cu:153-160).
+    const size_t plane = n * n * 24;
+    if (g_frame_dev != g_plan->device && g_frame.p) {  // a frame left on another device
+        scw::DeviceGuard g0(g_frame_dev);
+        g_frame.release();
+    }
+    scw::DeviceGuard g(g_plan->device);
+    if ((g_status = g.st) != SCW_OK) return;
+    if ((g_status = g_frame.alloc(3 * plane)) != SCW_OK) return;
+    g_frame_dev = g_plan->device;
+    unsigned char* f = g_frame.as<unsigned char>();
+    if ((g_status = scw_eval_planes(g_plan, x_start, y_start, z_start, x, y, z, d, N, 0.001, g_form,
+                                    g_form == SCW_FORM_DIRECT ? SCW_DTYPE_FP32 : g_dtype, f, f + plane, f + 2 * plane,
+                                    nullptr)) != SCW_OK)
+        return;
+    void* outs[3] = {E_xy, E_xz, E_yz};
+    for (int k = 0; k < 3; ++k) {
+        const hipError_t e = hipMemcpy(outs[k], f + k * plane, plane, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            g_status = scw::hip_fail(e, "hipMemcpy");
+            return;
+        }
+    }
 }
 
 void cpu_cw_evaluate_xy(scw_vec3c32* E_xy, float x_start, float y_start, float z, float d, size_t N) {

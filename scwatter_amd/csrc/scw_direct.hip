@@ -153,32 +153,51 @@ __global__ __launch_bounds__(256) void direct_reduce(const float* __restrict__ p
     out[i] = s;
 }
 
-int launch_direct(const scw_exec* ex, float* out, hipStream_t st) {
-    const scw_plan* P = ex->plan;
-    const int64_t pts = ex->nx * ex->ny;
-    const int ppt = ex->direct_ppt;
+int direct_ppt(int64_t pts) { return pts >= 1024 ? 4 : 1; }
+
+// Split the wave range until the grid has about 2048 workgroups, keeping at
+// least 1024 waves per split and at most 64 splits.
+int direct_split(int64_t pts, int64_t nz, int64_t maxw, int ppt) {
+    const int64_t bps = (pts + 256 * ppt - 1) / (256 * ppt);
+    const int64_t blocks = bps * nz;
+    int split = 1;
+    while (blocks * split < 2048 && maxw / (2 * split) >= 1024 && split < 64) split *= 2;
+    return split;
+}
+
+// One direct-form evaluation of an nx x ny x nz grid from device tables:
+// partial sums of the nsplit wave slices go to part (nsplit > 1) and are
+// added into out by a second kernel.
+int launch_direct_raw(const scw_plan* P, const void* region_table, const int32_t* zreg, const double* zs,
+                      const float* dxs, const float* dys, int64_t nx, int64_t ny, int64_t nz, double xc, double yc,
+                      int ppt, int nsplit, float* part, float* out, hipStream_t st) {
+    const DirectRegion* regions = static_cast<const DirectRegion*>(region_table);
+    const int64_t pts = nx * ny;
     const int64_t bps = (pts + (int64_t)DT * ppt - 1) / ((int64_t)DT * ppt);
-    const int nsplit = ex->direct_split;
-    float* dst = nsplit > 1 ? ex->d_C.as<float>() : out;
-    dim3 grid((unsigned)bps, (unsigned)ex->nz, (unsigned)nsplit);
+    float* dst = nsplit > 1 ? part : out;
+    dim3 grid((unsigned)bps, (unsigned)nz, (unsigned)nsplit);
     if (ppt == 4)
         hipLaunchKernelGGL(direct_kernel<4>, grid, dim3(DT), 0, st, P->d_k64.as<double>(), P->d_e64.as<double>(),
-                           P->d_lns.as<double>(), P->d_kf.as<float4>(), ex->d_regions.as<DirectRegion>(),
-                           ex->d_zslab.as<int32_t>(), ex->d_z.as<double>(), ex->d_dx.as<float>(),
-                           ex->d_dy.as<float>(), (int)ex->nx, (int)ex->ny, ex->xc, ex->yc, nsplit, dst);
+                           P->d_lns.as<double>(), P->d_kf.as<float4>(), regions, zreg, zs, dxs, dys, (int)nx, (int)ny,
+                           xc, yc, nsplit, dst);
     else
         hipLaunchKernelGGL(direct_kernel<1>, grid, dim3(DT), 0, st, P->d_k64.as<double>(), P->d_e64.as<double>(),
-                           P->d_lns.as<double>(), P->d_kf.as<float4>(), ex->d_regions.as<DirectRegion>(),
-                           ex->d_zslab.as<int32_t>(), ex->d_z.as<double>(), ex->d_dx.as<float>(),
-                           ex->d_dy.as<float>(), (int)ex->nx, (int)ex->ny, ex->xc, ex->yc, nsplit, dst);
+                           P->d_lns.as<double>(), P->d_kf.as<float4>(), regions, zreg, zs, dxs, dys, (int)nx, (int)ny,
+                           xc, yc, nsplit, dst);
     SCW_HIP(hipGetLastError());
     if (nsplit > 1) {
-        const int64_t n = pts * ex->nz * 6;
-        hipLaunchKernelGGL(direct_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                           (const float*)dst, nsplit, n, out);
+        const int64_t n = pts * nz * 6;
+        hipLaunchKernelGGL(direct_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (const float*)dst,
+                           nsplit, n, out);
         SCW_HIP(hipGetLastError());
     }
     return SCW_OK;
+}
+
+int launch_direct(const scw_exec* ex, float* out, hipStream_t st) {
+    return launch_direct_raw(ex->plan, ex->plan->d_dreg.p, ex->d_zslab.as<int32_t>(),
+                             ex->d_z.as<double>(), ex->d_dx.as<float>(), ex->d_dy.as<float>(), ex->nx, ex->ny, ex->nz,
+                             ex->xc, ex->yc, ex->direct_ppt, ex->direct_split, ex->d_C.as<float>(), out, st);
 }
 
 size_t direct_region_bytes() { return sizeof(DirectRegion); }
@@ -192,3 +211,4 @@ void direct_region_fill(void* dst, int64_t begin, int64_t end, int real_transver
 }
 
 }  // namespace scw
+

@@ -64,29 +64,20 @@ static int prep_direct(scw_exec* ex, const double* xs, const double* ys, const d
     for (int64_t i = 0; i < ex->nx; ++i) dx[i] = (float)(xs[i] - ex->xc);
     for (int64_t i = 0; i < ex->ny; ++i) dy[i] = (float)(ys[i] - ex->yc);
     std::vector<int32_t> zr(ex->zregion.begin(), ex->zregion.end());
-    std::vector<unsigned char> regs(direct_region_bytes() * (P->layers + 1));
     int64_t maxw = 0;
-    for (int r = 0; r <= P->layers; ++r) {
-        const RegionPlan& R = P->regions[r];
-        direct_region_fill(&regs[direct_region_bytes() * r], R.wbegin, R.wend, R.real_transverse ? 1 : 0);
-    }
     for (int64_t z = 0; z < ex->nz; ++z) {
         const RegionPlan& R = P->regions[ex->zregion[z]];
         maxw = std::max(maxw, R.wend - R.wbegin);
     }
     const int64_t pts = ex->nx * ex->ny;
-    ex->direct_ppt = pts >= 1024 ? 4 : 1;
-    const int64_t bps = (pts + 256 * ex->direct_ppt - 1) / (256 * ex->direct_ppt);
-    const int64_t blocks = bps * ex->nz;
-    int split = 1;
-    while (blocks * split < 2048 && maxw / (2 * split) >= 1024 && split < 64) split *= 2;
+    ex->direct_ppt = direct_ppt(pts);
+    const int split = direct_split(pts, ex->nz, maxw, ex->direct_ppt);
     ex->direct_split = split;
     int st;
     if ((st = ex->d_dx.upload(dx.data(), dx.size() * 4)) != SCW_OK) return st;
     if ((st = ex->d_dy.upload(dy.data(), dy.size() * 4)) != SCW_OK) return st;
     if ((st = ex->d_z.upload(zs, ex->nz * sizeof(double))) != SCW_OK) return st;
     if ((st = ex->d_zslab.upload(zr.data(), zr.size() * 4)) != SCW_OK) return st;
-    if ((st = ex->d_regions.upload(regs.data(), regs.size())) != SCW_OK) return st;
     if (split > 1 && (st = ex->d_C.alloc((size_t)split * pts * ex->nz * 24)) != SCW_OK) return st;
     ex->flops = 36.0 * ex->units;
     ex->bytes = 24.0 * (double)pts * ex->nz;
@@ -647,16 +638,13 @@ static void exec_free(scw_exec* ex) {
     delete ex;
 }
 
-}  // namespace scw
-
-using namespace scw;
-
-extern "C" {
-
-int scw_exec_create(scw_plan* plan, const double* xs, size_t nx, const double* ys, size_t ny, const double* zs,
-                    size_t nz, double region_tol, int form, int dtype, scw_exec** out) {
-    if (!out) return fail(SCW_ERR_ARG, "exec output is NULL");
-    *out = nullptr;
+// (Re)builds ex for a single plan on the grid xs x ys x zs: validation, form
+// choice, tables.  An exec being rebuilt keeps its device buffers where they
+// are large enough (DevBuf capacity), so a rebuild of a similar grid needs no
+// allocation; a captured graph is dropped.  On failure the exec is left
+// invalid (runs refused) and the caller destroys or rebuilds it.
+int exec_init(scw_exec* ex, scw_plan* plan, const double* xs, size_t nx, const double* ys, size_t ny,
+              const double* zs, size_t nz, double region_tol, int form, int dtype) {
     if (!plan) return fail(SCW_ERR_ARG, "plan is NULL");
     if (!xs || !ys || !zs) return fail(SCW_ERR_ARG, "coordinate array is NULL");
     if (nx == 0 || ny == 0 || nz == 0) return fail(SCW_ERR_ARG, "empty grid (%zu x %zu x %zu)", nx, ny, nz);
@@ -672,65 +660,92 @@ int scw_exec_create(scw_plan* plan, const double* xs, size_t nx, const double* y
         if (!std::isfinite(ys[i])) return fail(SCW_ERR_ARG, "ys[%zu] is not finite", i);
     for (size_t i = 0; i < nz; ++i)
         if (!std::isfinite(zs[i])) return fail(SCW_ERR_ARG, "zs[%zu] is not finite", i);
+    if (ex->plan && ex->plan != plan) return fail(SCW_ERR_ARG, "internal: exec rebuilt for another plan");
 
-    scw_exec* ex = new (std::nothrow) scw_exec();
-    if (!ex) return fail(SCW_ERR_NOMEM, "host allocation failed");
+    DeviceGuard g(plan->device);
+    if (g.st != SCW_OK) return g.st;
+    ex->invalid = true;  // until the tables below are complete
+    if (ex->graph_exec) {
+        (void)hipGraphExecDestroy(ex->graph_exec);
+        ex->graph_exec = nullptr;
+    }
+    if (ex->graph) {
+        (void)hipGraphDestroy(ex->graph);
+        ex->graph = nullptr;
+    }
     ex->plan = plan;
+    ex->batch.clear();
     ex->nx = (int64_t)nx;
     ex->ny = (int64_t)ny;
     ex->nz = (int64_t)nz;
     ex->dtype = dtype;
-    ex->zregion.resize(nz);
-    bool all_sep = true;
-    int first_bad = -1;
-    double units = 0.0;
-    for (size_t z = 0; z < nz; ++z) {
-        const int r = region_of(zs[z], plan->zl, region_tol);
-        ex->zregion[z] = r;
-        const RegionPlan& R = plan->regions[r];
-        units += (double)nx * ny * (double)(R.wend - R.wbegin);
-        if (!R.separable && all_sep) {
-            all_sep = false;
-            first_bad = r;
-        }
-    }
-    ex->units = units;
-    const bool auto_form = form == SCW_FORM_AUTO;
-    if (form == SCW_FORM_AUTO) form = all_sep ? SCW_FORM_GEMM : SCW_FORM_DIRECT;
-    if (form == SCW_FORM_GEMM && !all_sep) {
-        const int st = fail(SCW_ERR_UNSUPPORTED, "GEMM form: region %d is not separable (%s)", first_bad,
-                            plan->regions[first_bad].why_not.c_str());
-        delete ex;
-        return st;
-    }
-    ex->form = form;
-    if (form == SCW_FORM_DIRECT) ex->dtype = SCW_DTYPE_FP32;
-
-    DeviceGuard g(plan->device);
-    if (g.st != SCW_OK) {
-        delete ex;
-        return g.st;
-    }
     int st;
     try {
+        ex->zregion.resize(nz);
+        bool all_sep = true;
+        int first_bad = -1;
+        double units = 0.0;
+        for (size_t z = 0; z < nz; ++z) {
+            const int r = region_of(zs[z], plan->zl, region_tol);
+            ex->zregion[z] = r;
+            const RegionPlan& R = plan->regions[r];
+            units += (double)nx * ny * (double)(R.wend - R.wbegin);
+            if (!R.separable && all_sep) {
+                all_sep = false;
+                first_bad = r;
+            }
+        }
+        ex->units = units;
+        const bool auto_form = form == SCW_FORM_AUTO;
+        if (form == SCW_FORM_AUTO) form = all_sep ? SCW_FORM_GEMM : SCW_FORM_DIRECT;
+        if (form == SCW_FORM_GEMM && !all_sep)
+            return fail(SCW_ERR_UNSUPPORTED, "GEMM form: region %d is not separable (%s)", first_bad,
+                        plan->regions[first_bad].why_not.c_str());
+        ex->form = form;
+        if (form == SCW_FORM_DIRECT) ex->dtype = SCW_DTYPE_FP32;
         ex->hx.assign(xs, xs + nx);
         ex->hy.assign(ys, ys + ny);
         ex->hz.assign(zs, zs + nz);
         ex->hzp.assign(nz, 0);
-        st = form == SCW_FORM_DIRECT ? prep_direct(ex, xs, ys, zs) : prep_gemm(ex, xs, ys, entries(ex));
-        if (st == SCW_ERR_UNSUPPORTED && auto_form && form == SCW_FORM_GEMM) {
-            // Separable, but this grid takes a phase table past FP32's range
-            // (a point far from the region's phase origin in a strongly
-            // evanescent mode) or a U slice past 32-bit offsets: AUTO falls
-            // back to the direct form, which carries such exponents in FP64.
+        ex->nslots_max = 1;
+        if (form == SCW_FORM_DIRECT) {
             release_gemm(ex);
-            ex->form = SCW_FORM_DIRECT;
-            ex->dtype = SCW_DTYPE_FP32;
             st = prep_direct(ex, xs, ys, zs);
+        } else {
+            for (DevBuf* b : {&ex->d_dx, &ex->d_dy, &ex->d_z, &ex->d_zslab}) b->release();
+            st = prep_gemm(ex, xs, ys, entries(ex));
+            if (st == SCW_ERR_UNSUPPORTED && auto_form) {
+                // Separable, but this grid takes a phase table past FP32's
+                // range (a point far from the region's phase origin in a
+                // strongly evanescent mode) or a U slice past 32-bit offsets:
+                // AUTO falls back to the direct form, which carries such
+                // exponents in FP64.
+                release_gemm(ex);
+                ex->form = SCW_FORM_DIRECT;
+                ex->dtype = SCW_DTYPE_FP32;
+                st = prep_direct(ex, xs, ys, zs);
+            }
         }
     } catch (const std::bad_alloc&) {
         st = fail(SCW_ERR_NOMEM, "host allocation failed while building grid tables");
     }
+    if (st == SCW_OK) ex->invalid = false;
+    return st;
+}
+
+}  // namespace scw
+
+using namespace scw;
+
+extern "C" {
+
+int scw_exec_create(scw_plan* plan, const double* xs, size_t nx, const double* ys, size_t ny, const double* zs,
+                    size_t nz, double region_tol, int form, int dtype, scw_exec** out) {
+    if (!out) return fail(SCW_ERR_ARG, "exec output is NULL");
+    *out = nullptr;
+    scw_exec* ex = new (std::nothrow) scw_exec();
+    if (!ex) return fail(SCW_ERR_NOMEM, "host allocation failed");
+    const int st = exec_init(ex, plan, xs, nx, ys, ny, zs, nz, region_tol, form, dtype);
     if (st != SCW_OK) {
         exec_free(ex);
         return st;

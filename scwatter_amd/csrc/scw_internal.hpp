@@ -39,10 +39,13 @@ struct DeviceGuard {
     }
 };
 
-// Device buffer owned by a plan or exec.
+// Device buffer owned by a plan or exec.  alloc / upload keep the current
+// allocation when it is large enough (a rebuilt table of the same or smaller
+// size then needs no hipMalloc); release frees it.
 struct DevBuf {
     void* p = nullptr;
-    size_t bytes = 0;
+    size_t bytes = 0;  // size of the current contents
+    size_t cap = 0;    // size of the allocation
     int alloc(size_t n);
     int upload(const void* host, size_t n);
     void release();
@@ -91,6 +94,17 @@ struct scw_plan {
     // 16-bit exec that needs it (scw::build_bfp).
     scw::DevBuf d_bfp;
     std::vector<int64_t> bfp_off;  // byte offset per region, -1 if none
+    // direct form, per region: wave range and whether the transverse k is real
+    // (scw::direct_region_fill records), and the boundaries as double[layers]
+    scw::DevBuf d_dreg, d_zl;
+    // execs of recent scw_eval_planes calls, most recent last (scw_planes.cpp)
+    struct PlaneEntry {
+        int kind, n, form, dtype;
+        float start_a, start_b, fixed, d;
+        double tol;
+        scw_exec* ex;
+    };
+    std::vector<PlaneEntry> plane_cache;
 };
 
 namespace scw {
@@ -205,8 +219,22 @@ struct scw_exec {
 namespace scw {
 // kernel launchers (scw_direct.hip, scw_gemm.hip)
 int launch_direct(const scw_exec* ex, float* out, hipStream_t s);
+// the direct form's wave-range split for a grid of pts points per slice, nz
+// slices and at most maxw waves per region (shared by execs and the
+// allocation-free plane evaluation, so both sum in the same order)
+int direct_split(int64_t pts, int64_t nz, int64_t maxw, int ppt);
+int direct_ppt(int64_t pts);
+// One direct-form evaluation of an nx x ny x nz grid from device tables
+// (region_table: the plan's d_dreg records): partial sums of the nsplit wave
+// slices go to part (nsplit > 1) and are added into out by a second kernel.
+int launch_direct_raw(const scw_plan* P, const void* region_table, const int32_t* zreg, const double* zs,
+                      const float* dxs, const float* dys, int64_t nx, int64_t ny, int64_t nz, double xc, double yc,
+                      int ppt, int nsplit, float* part, float* out, hipStream_t st);
 int launch_gemm(const scw_exec* ex, float* out, hipStream_t s, int stage = 0);
 int build_bfp(scw_plan* plan);  // fills plan->d_bfp / bfp_off once (scw_gemm.hip)
+// (re)builds a single-plan exec in place (scw_exec.cpp); on failure it is left invalid
+int exec_init(scw_exec* ex, scw_plan* plan, const double* xs, size_t nx, const double* ys, size_t ny,
+              const double* zs, size_t nz, double region_tol, int form, int dtype);
 // dst = the exponent plane src (n words, erow per mode group) with the byte
 // of mode c raised by shift[c] (device pointers; scw_gemm.hip)
 int shift_bfp_exponents(const uint8_t* src, uint8_t* dst, int64_t erow, int64_t n, const int32_t* shift);

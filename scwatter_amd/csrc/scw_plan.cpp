@@ -44,21 +44,31 @@ int hip_fail(hipError_t e, const char* what) {
 }
 
 // Allocation-failure injection for tests of the error paths: after
-// scw_test_fail_alloc_after(k), the (k+1)-th device allocation from then on
-// fails as out-of-memory (k < 0 turns it off).
+// scw_test_fail_alloc_after(k), the (k+1)-th non-empty buffer request from
+// then on fails as out-of-memory (k < 0 turns it off).
 static std::atomic<int64_t> g_fail_after{-1};
 
 int DevBuf::alloc(size_t n) {
-    release();
-    if (n == 0) return SCW_OK;
-    if (g_fail_after.load() >= 0 && g_fail_after.fetch_sub(1) == 0)
+    if (n == 0) {
+        release();
+        return SCW_OK;
+    }
+    // every non-empty request counts for the injected failures, reused or not
+    if (g_fail_after.load() >= 0 && g_fail_after.fetch_sub(1) == 0) {
+        release();
         return fail(SCW_ERR_NOMEM, "hipMalloc: injected allocation failure (%zu bytes)", n);
+    }
+    if (p && n <= cap) {
+        bytes = n;
+        return SCW_OK;
+    }
+    release();
     hipError_t e = hipMalloc(&p, n);
     if (e != hipSuccess) {
         p = nullptr;
         return hip_fail(e, "hipMalloc");
     }
-    bytes = n;
+    bytes = cap = n;
     return SCW_OK;
 }
 
@@ -73,7 +83,7 @@ int DevBuf::upload(const void* host, size_t n) {
 void DevBuf::release() {
     if (p) (void)hipFree(p);
     p = nullptr;
-    bytes = 0;
+    bytes = cap = 0;
 }
 
 // Group values that agree to within tol in both parts; returns distinct
@@ -389,6 +399,10 @@ int scw_plan_create(int device, const scw_planewave64* waves, size_t n_waves, co
         }
     }
     P->ell_elems = slotsA;
+    std::vector<unsigned char> dreg(direct_region_bytes() * (layers + 1));
+    for (int r = 0; r <= layers; ++r)
+        direct_region_fill(&dreg[direct_region_bytes() * r], P->regions[r].wbegin, P->regions[r].wend,
+                           P->regions[r].real_transverse ? 1 : 0);
 
     int st = SCW_OK;
     if ((st = P->d_k64.upload(k64.data(), k64.size() * sizeof(double))) != SCW_OK ||
@@ -396,8 +410,10 @@ int scw_plan_create(int device, const scw_planewave64* waves, size_t n_waves, co
         (st = P->d_lns.upload(lns.data(), lns.size() * sizeof(double))) != SCW_OK ||
         (st = P->d_kf.upload(kf.data(), kf.size() * sizeof(float))) != SCW_OK ||
         (st = P->d_ellA.upload(ellA.data(), ellA.size() * sizeof(float))) != SCW_OK ||
-        (st = P->d_ellC.upload(ellC.data(), ellC.size() * sizeof(int32_t))) != SCW_OK) {
-        delete P;
+        (st = P->d_ellC.upload(ellC.data(), ellC.size() * sizeof(int32_t))) != SCW_OK ||
+        (st = P->d_dreg.upload(dreg.data(), dreg.size())) != SCW_OK ||
+        (st = P->d_zl.upload(P->zl.data(), P->zl.size() * sizeof(double))) != SCW_OK) {
+        scw_plan_destroy(P);
         return st;
     }
     *out = P;
@@ -408,6 +424,10 @@ int scw_plan_destroy(scw_plan* plan) {
     if (!plan) return SCW_OK;
     {
         DeviceGuard g(plan->device);
+        for (auto& e : plan->plane_cache) scw_exec_destroy(e.ex);
+        plan->plane_cache.clear();
+        plan->d_dreg.release();
+        plan->d_zl.release();
         plan->d_k64.release();
         plan->d_e64.release();
         plan->d_lns.release();
@@ -444,7 +464,7 @@ int scw_plan_region_info(const scw_plan* plan, int region, int64_t* n_waves, int
 int scw_plan_device_bytes(const scw_plan* plan, size_t* bytes) {
     if (!plan || !bytes) return fail(SCW_ERR_ARG, "NULL argument");
     *bytes = plan->d_k64.bytes + plan->d_e64.bytes + plan->d_lns.bytes + plan->d_kf.bytes +
-             plan->d_ellA.bytes + plan->d_ellC.bytes + plan->d_bfp.bytes;
+             plan->d_ellA.bytes + plan->d_ellC.bytes + plan->d_bfp.bytes + plan->d_dreg.bytes + plan->d_zl.bytes;
     return SCW_OK;
 }
 

@@ -181,20 +181,30 @@ def axis_gpu(start, d, N):
     return (np.float64(np.float32(d)) * i + np.float64(np.float32(start))).astype(np.float32).astype(np.float64)
 
 
+def eval_planes(plan: Plan, x_start, y_start, z_start, x, y, z, d, N, out_xy, out_xz, out_yz, region_tol=0.001,
+                form="auto", dtype="fp32", stream=None):
+    """scw_eval_planes: the three N x N planes of gpu_cw_evaluate into device
+    memory (out_* device pointers, 0/None skips a plane), asynchronously."""
+    f = lambda v: float(np.float32(v))
+    check(_native.lib().scw_eval_planes(plan.handle, f(x_start), f(y_start), f(z_start), f(x), f(y), f(z), f(d),
+                                        int(N), float(region_tol), _native.FORMS[form], _native.DTYPES[dtype],
+                                        out_xy or None, out_xz or None, out_yz or None, stream))
+
+
 def gpu_cw_evaluate(plan: Plan, x_start, y_start, z_start, x, y, z, d, N, form="auto", dtype="fp32"):
     """gpu_cw_evaluate (gpuEvaluator.cu:131-165): xy plane at z, xz plane at y, yz plane at x.
 
-    Region rule z >= z_l - 0.001 (gpuEvaluator.cu:34).  Returns three complex64
-    (N, N, 3) arrays: E_xy[iy, ix], E_xz[iz, ix], E_yz[iz, iy].
+    Region rule z >= z_l - 0.001 (gpuEvaluator.cu:34), coordinates start + d i
+    fused (gpuEvaluator.cu:78-79); through scw_eval_planes.  Returns three
+    complex64 (N, N, 3) arrays: E_xy[iy, ix], E_xz[iz, ix], E_yz[iz, iy].
     """
-    xs, ys, zs = axis_gpu(x_start, d, N), axis_gpu(y_start, d, N), axis_gpu(z_start, d, N)
-    f32 = lambda v: [float(np.float32(v))]
-    out = []
-    for gx, gy, gz in ((xs, ys, f32(z)), (xs, f32(y), zs), (f32(x), ys, zs)):
-        ex = Exec(plan, gx, gy, gz, 0.001, form, dtype)
-        out.append(ex.run_host().reshape(N, N, 3))
-        ex.destroy()
-    return tuple(out)
+    from . import hip
+    bufs = [hip.DeviceArray((N, N, 3), np.complex64) for _ in range(3)]
+    eval_planes(plan, x_start, y_start, z_start, x, y, z, d, N, *(b.ptr for b in bufs), form=form, dtype=dtype)
+    out = tuple(b.to_host() for b in bufs)
+    for b in bufs:
+        b.free()
+    return out
 
 
 def cpu_cw_evaluate(kind, plan: Plan, a_start, b_start, fixed, d, N, form="auto", dtype="fp32"):
