@@ -169,6 +169,22 @@ int scw_eval_planes(scw_plan* plan, float x_start, float y_start, float z_start,
                     float d, int N, double region_tol, int form, int dtype, void* E_xy, void* E_xz, void* E_yz,
                     void* stream);
 
+/* Allocation-free variant of scw_eval_planes' direct form.  No counterpart in
+ * the reference (its gpu_cw_evaluate allocates and frees a device frame on
+ * every call, gpuEvaluator.cu:137-163); it is for callers that cannot
+ * allocate device memory: code under HIP graph capture, tight serving or GUI
+ * loops, harnesses that forbid hipMalloc.  Same arguments, coordinates,
+ * region rule and layouts as scw_eval_planes with form SCW_FORM_DIRECT; the
+ * planes are bit-identical to it.  The per-plane tables and the partial sums
+ * of the wave-range split live in one fixed 16 MiB device array per device,
+ * held until the process ends; N is limited to 16384 (larger N returns
+ * SCW_ERR_ARG), and the split is reduced where its partial sums would not fit
+ * (the sums are then still correct, but no longer bit-identical).  Calls on
+ * one device share that array: calls that overlap on one device race. */
+int scw_eval_planes_noalloc(const scw_plan* plan, float x_start, float y_start, float z_start, float x, float y,
+                            float z, float d, int N, double region_tol, void* E_xy, void* E_xz, void* E_yz,
+                            void* stream);
+
 /* ---- On-device post-processing of fields (sCWatterView.cpp:194-352) -----
  * Device buffers, asynchronous on `stream`, no allocation.  Scalars and
  * colours are bit-identical to the reference's float arithmetic; min/max

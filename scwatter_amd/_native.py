@@ -27,6 +27,7 @@ EXPORTS = (
     "scw_cw_load", "scw_cw_save", "scw_cw_create", "scw_cw_destroy", "scw_cw_set_incident", "scw_cw_add_layer",
     "scw_cw_counts", "scw_cw_layer", "scw_cw_unpack",
     "scw_post_scalar", "scw_post_minmax", "scw_post_color", "scw_test_fail_alloc_after", "scw_eval_planes",
+    "scw_eval_planes_noalloc",
 )
 
 
@@ -91,6 +92,8 @@ def lib():
         "scw_test_fail_alloc_after": (None, [ctypes.c_int64]),
         "scw_eval_planes": (ctypes.c_int, [_vp] + [ctypes.c_float] * 7 + [ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                                                           ctypes.c_int, _vp, _vp, _vp, _vp]),
+        "scw_eval_planes_noalloc": (ctypes.c_int, [_vp] + [ctypes.c_float] * 7 + [ctypes.c_int, ctypes.c_double, _vp,
+                                                                                  _vp, _vp, _vp]),
         "scw_post_scalar": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_int, _vp, _vp]),
         "scw_post_minmax": (ctypes.c_int, [_vp, ctypes.c_size_t, _vp, ctypes.c_int, _vp]),
         "scw_post_color": (ctypes.c_int, [_vp, ctypes.c_size_t, _vp, ctypes.POINTER(ctypes.c_float),

@@ -191,6 +191,17 @@ def eval_planes(plan: Plan, x_start, y_start, z_start, x, y, z, d, N, out_xy, ou
                                         out_xy or None, out_xz or None, out_yz or None, stream))
 
 
+def eval_planes_noalloc(plan: Plan, x_start, y_start, z_start, x, y, z, d, N, out_xy, out_xz, out_yz,
+                        region_tol=0.001, stream=None):
+    """scw_eval_planes_noalloc: eval_planes' direct form with no device
+    allocation (tables and partial sums in the library's fixed device array;
+    N <= 16384)."""
+    f = lambda v: float(np.float32(v))
+    check(_native.lib().scw_eval_planes_noalloc(plan.handle, f(x_start), f(y_start), f(z_start), f(x), f(y), f(z),
+                                                f(d), int(N), float(region_tol), out_xy or None, out_xz or None,
+                                                out_yz or None, stream))
+
+
 def gpu_cw_evaluate(plan: Plan, x_start, y_start, z_start, x, y, z, d, N, form="auto", dtype="fp32"):
     """gpu_cw_evaluate (gpuEvaluator.cu:131-165): xy plane at z, xz plane at y, yz plane at x.
 
