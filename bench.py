@@ -227,6 +227,8 @@ def main():
                          "10x larger")
     ap.add_argument("--dtypes", default="fp32,bf16x3,bf16,fp16")
     ap.add_argument("--workload", default="c4", choices=sorted(WORKLOADS))
+    ap.add_argument("--physical", action="store_true", help="tile sample (c2): eigen-tables from the host sample "
+                    "solver instead of seeded synthetic ones")
     ap.add_argument("--no-drop-in", action="store_true", help="leave out the drop-in latency measurement")
     ap.add_argument("--retune", action="store_true", help="time the launch-shape candidates even if the "
                     "tuning table has an entry for the run")
@@ -255,6 +257,11 @@ def main():
         return bench_batch(args)
 
     desc, gen = WORKLOADS[args.workload]
+    if args.physical:
+        if args.workload != "c2":
+            print("bench.py: --physical applies to the tile sample (--workload c2)", file=sys.stderr)
+            sys.exit(2)
+        desc, gen = desc + "; physical eigen-tables (host sample solver)", "config2_physical"
     t0 = time.perf_counter()
     cw, (xs, ys, zs) = getattr(configs, gen)()
     tables = cw.unpack()

@@ -312,3 +312,31 @@ def map_instances(fn, params, workers=8):
         return [one(i) for i in range(len(params))]
     with ThreadPoolExecutor(max_workers=workers) as pool:
         return list(pool.map(one, range(len(params))))
+
+
+# ---------------------------------------------------------------------------
+# Physical eigen-tables: the same slab stacks solved by the host pipeline
+# (sample_solver, sCWatter's Phi / eigen-solve / boundary chain) instead of
+# seeded synthetic eigenvalues and amplitudes.
+# ---------------------------------------------------------------------------
+
+def config2_physical(nx=256, ny=256, nz=128, M=21, Z=20, pixels=64, tile=8, seed=None, workers=None):
+    """Tile-sample slab stack 64 x 64 x 20 um, Z sublayers of pixels^2 voxels in
+    seeded random tile x tile tiles of n in {1.0, 1.4 + 0.01i}, lit at normal
+    incidence by a y-polarised plane wave at lambda 1.0 um, M x M orders,
+    solved by sample_solver; the grid of config2."""
+    from .sample_solver import solve_sample, tile_sample
+    n = tile_sample(Z, pixels, pixels, tile=tile, n_background=1.0, n_tile=1.4 + 0.01j,
+                    seed=BASE_SEED + 21 if seed is None else seed)
+    cw = solve_sample(n, 1.0, (64.0, 64.0, 20.0), 0.0, (M, M), (0.0, 0.0, 1.0), (0.0, 1.0, 0.0), workers=workers)
+    return cw, (axis(-32.0, 32.0, nx), axis(-32.0, 32.0, ny), axis(-10.0, 10.0, nz))
+
+
+def config4_physical_structure(p, pixels=64, tile=8, workers=None):
+    """One instance of the batched sweep (its lambda, incidence angle and order
+    count) as a physical single-slab tile sample of the sweep's size."""
+    from .sample_solver import solve_sample, tile_sample
+    n = tile_sample(1, pixels, pixels, tile=tile, n_background=1.0, n_tile=1.4 + 0.01j, seed=p["seed"])
+    th = math.radians(p["angle"])
+    return solve_sample(n, p["lam"], p["size"], 0.0, (p["M"], p["M"]), (math.sin(th), 0.0, math.cos(th)),
+                        (0.0, 1.0, 0.0), workers=workers)
