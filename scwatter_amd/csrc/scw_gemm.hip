@@ -78,6 +78,46 @@ __device__ inline float2 gload(const float2* p) {
     return make_float2(v[0], v[1]);
 }
 
+// Partial sums handed from one workgroup to another inside the single-launch
+// form (fused16) go through agent-scope relaxed atomic accesses, which gfx950
+// issues with sc1: stores write through and loads read past the L2 of their
+// XCD, so a consumer on another XCD sees what the producer wrote once the
+// producer's stores are complete (s_waitcnt vmcnt(0)) and it has raised the
+// counter the consumer polls.  COH = false is the ordinary cached access of
+// the stage-by-stage kernels.
+template <bool COH>
+__device__ inline void put2(float2* p, const float2 v) {
+    if constexpr (COH) {
+        const uint64_t u = __builtin_bit_cast(uint64_t, make_float2(v.x, v.y));
+        __hip_atomic_store(reinterpret_cast<uint64_t*>(p), u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        *p = v;
+    }
+}
+template <bool COH>
+__device__ inline float2 get2(const float2* p) {
+    if constexpr (COH) {
+        const uint64_t u = __hip_atomic_load(reinterpret_cast<uint64_t*>(const_cast<float2*>(p)), __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+        return __builtin_bit_cast(float2, u);
+    } else {
+        return *p;
+    }
+}
+// Two consecutive float2 as one floatx4 (16-byte aligned).
+template <bool COH>
+__device__ inline floatx4 get4(const float2* p) {
+    if constexpr (COH) {
+        const float2 a = get2<true>(p), b = get2<true>(p + 1);
+        return floatx4{a.x, a.y, b.x, b.y};
+    } else {
+        return *reinterpret_cast<const floatx4*>(p);
+    }
+}
+// Waits until this thread's stores have completed (vmcnt 0): with sc1
+// stores, they are then visible to every XCD.
+__device__ inline void stores_done() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // ---------------------------------------------------------------------------
 // C_z[a,b,comp] partial sums over a term slice, for up to 8 z of one region.
 // One wave per work item; the item, region and Z-row offsets are wave-uniform
@@ -180,14 +220,11 @@ __device__ inline void xcd_block3(int& bx, int& by, int& bz) {
     bz = l / (gx * gy);
 }
 
-__global__ __launch_bounds__(256) void coef_kernel(const CoefItem* __restrict__ items, int nitems,
-                                                   const RegionDev* __restrict__ regions,
-                                                   const ZMeta* __restrict__ zmeta, const float2* __restrict__ Z,
-                                                   int64_t c_elems, float2* __restrict__ C) {
+template <bool COH>
+__device__ inline void coef_body(const CoefItem& I, const RegionDev* __restrict__ regions,
+                                 const ZMeta* __restrict__ zmeta, const float2* __restrict__ Z, int64_t c_elems,
+                                 float2* __restrict__ C) {
     const int lane = threadIdx.x & 63;
-    const int it = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
-    if (it >= nitems) return;
-    const CoefItem I = items[it];
     const RegionDev R = regions[I.region];
     const int cell = I.cell0 + lane;
     const bool active = cell < R.cells;
@@ -229,11 +266,20 @@ __global__ __launch_bounds__(256) void coef_kernel(const CoefItem* __restrict__ 
     for (int zi = 0; zi < 8; ++zi) {
         if (zi < nz) {
             float2* o = C + (int64_t)I.slot * c_elems + zmeta[I.z0 + zi].c_off + (int64_t)cell * 3;
-            o[0] = make_float2(acc[zi][0], acc[zi][1]);
-            o[1] = make_float2(acc[zi][2], acc[zi][3]);
-            o[2] = make_float2(acc[zi][4], acc[zi][5]);
+            put2<COH>(o, make_float2(acc[zi][0], acc[zi][1]));
+            put2<COH>(o + 1, make_float2(acc[zi][2], acc[zi][3]));
+            put2<COH>(o + 2, make_float2(acc[zi][4], acc[zi][5]));
         }
     }
+}
+
+__global__ __launch_bounds__(256) void coef_kernel(const CoefItem* __restrict__ items, int nitems,
+                                                   const RegionDev* __restrict__ regions,
+                                                   const ZMeta* __restrict__ zmeta, const float2* __restrict__ Z,
+                                                   int64_t c_elems, float2* __restrict__ C) {
+    const int it = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+    if (it >= nitems) return;
+    coef_body<false>(items[it], regions, zmeta, Z, c_elems, C);
 }
 
 constexpr int MX_MAX = 128;
@@ -560,6 +606,42 @@ __device__ inline floatx4 mfma16(const typename Op16<MODE>::vec a, const typenam
 // into bf16 hi + lo (bf16 x3: hi hi + hi lo + lo hi), as the FP32 X
 // contraction kernel did before it moved here.
 // ---------------------------------------------------------------------------
+// Element e = m ka + a of the slot-summed Cs_z of one z (m < 3 my, a < ka).
+template <bool COH>
+__device__ inline float2 csum_elem(const ZMeta& zm, const RegionDev& R, const float2* __restrict__ C,
+                                   int64_t c_elems, const float2* __restrict__ Z, int e) {
+    const int ka = R.ka, ns = R.nslots;
+    const int m = e / ka, a = e - m * ka;
+    float2 sum = make_float2(0.0f, 0.0f);
+    if (a >= zm.mx) return sum;
+    if (ns == 0) {
+        // a region of few terms per cell, not contracted before: sum its terms
+        // here, C_z[cell][comp] = sum_j A[j][cell][comp] Z_z[kz index of term j]
+        const int b = m / 3, comp = m - 3 * b, cell = b * zm.mx + a;
+        for (int j = 0; j < R.width; ++j) {
+            const float2 av = R.ellA[(int64_t)j * R.arow + (int64_t)cell * 3 + comp];
+            const float2 zv = Z[zm.zrow_off + R.ellC[(int64_t)j * R.cells + cell]];
+            sum.x += av.x * zv.x - av.y * zv.y;
+            sum.y += av.x * zv.y + av.y * zv.x;
+        }
+        return sum;
+    }
+    const int64_t off = zm.c_off + ((int64_t)(m / 3) * zm.mx + a) * 3 + (m % 3);
+    for (int k0 = 0; k0 < ns; k0 += 16) {
+        float2 v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            v[u] = k0 + u < ns ? get2<COH>(C + (int64_t)(k0 + u) * c_elems + off) : make_float2(0.0f, 0.0f);
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            if (k0 + u < ns) {
+                sum.x += v[u].x;
+                sum.y += v[u].y;
+            }
+    }
+    return sum;
+}
+
 __global__ __launch_bounds__(256) void csum_kernel(const ZMeta* __restrict__ zmeta,
                                                    const RegionDev* __restrict__ regions,
                                                    const float2* __restrict__ C, int64_t c_elems,
@@ -567,43 +649,10 @@ __global__ __launch_bounds__(256) void csum_kernel(const ZMeta* __restrict__ zme
                                                    const int2* __restrict__ blocks) {
     const int2 blk = blocks[blockIdx.x];  // (z, first element): only the elements each z has
     const ZMeta zm = zmeta[blk.x];
-    const int ka = regions[zm.region].ka, ns = regions[zm.region].nslots;
+    const RegionDev& R = regions[zm.region];
     const int e = blk.y + (int)threadIdx.x;
-    if (e >= 3 * zm.my * ka) return;
-    const int m = e / ka, a = e - m * ka;
-    float2 sum = make_float2(0.0f, 0.0f);
-    if (ns == 0) {
-        // a region of few terms per cell, not contracted before: sum its terms
-        // here, C_z[cell][comp] = sum_j A[j][cell][comp] Z_z[kz index of term j]
-        if (a < zm.mx) {
-            const RegionDev& R = regions[zm.region];
-            const int b = m / 3, comp = m - 3 * b, cell = b * zm.mx + a;
-            for (int j = 0; j < R.width; ++j) {
-                const float2 av = R.ellA[(int64_t)j * R.arow + (int64_t)cell * 3 + comp];
-                const float2 zv = Z[zm.zrow_off + R.ellC[(int64_t)j * R.cells + cell]];
-                sum.x += av.x * zv.x - av.y * zv.y;
-                sum.y += av.x * zv.y + av.y * zv.x;
-            }
-        }
-        Cs[zm.u_off + e] = sum;
-        return;
-    }
-    if (a < zm.mx) {
-        const int64_t off = zm.c_off + ((int64_t)(m / 3) * zm.mx + a) * 3 + (m % 3);
-        for (int k0 = 0; k0 < ns; k0 += 16) {
-            float2 v[16];
-#pragma unroll
-            for (int u = 0; u < 16; ++u)
-                v[u] = k0 + u < ns ? C[(int64_t)(k0 + u) * c_elems + off] : make_float2(0.0f, 0.0f);
-#pragma unroll
-            for (int u = 0; u < 16; ++u)
-                if (k0 + u < ns) {
-                    sum.x += v[u].x;
-                    sum.y += v[u].y;
-                }
-        }
-    }
-    Cs[zm.u_off + e] = sum;
+    if (e >= 3 * zm.my * R.ka) return;
+    Cs[zm.u_off + e] = csum_elem<false>(zm, R, C, c_elems, Z, e);
 }
 
 constexpr int UPL = 130;  // LDS row pitch (float2) of a workgroup's U_z columns
@@ -661,7 +710,7 @@ __device__ inline void xc_mfma(floatx4& acc, const bf16x8& ah, const bf16x8& al,
 // first stages the X fragments, already split, in LDS (xs), and each row tile
 // issues all its C loads at once: one L2 round trip per row tile instead of
 // one per K step; larger mx takes the K steps one at a time.
-template <int XS>
+template <int XS, bool COH>
 __device__ __attribute__((always_inline)) inline void xcontract_columns(const ZMeta& zm, const RegionDev& R, const float2* __restrict__ Cs,
                                          const float2* __restrict__ X, int nx, int bx, float2 (*Ul)[UPL], XFrag* xs) {
     const int lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4, o = j & 1;
@@ -712,10 +761,9 @@ __device__ __attribute__((always_inline)) inline void xcontract_columns(const ZM
             floatx4 cv[XS > 0 ? XS : 1][2];
 #pragma unroll
             for (int t = 0; t < XS; ++t) {  // unconditional: steps past T re-read the last one
-                const floatx4* cp =
-                    reinterpret_cast<const floatx4*>(Cz + (int64_t)m * R.ka + 16 * min(t, T - 1) + 4 * g);
-                cv[t][0] = cp[0];
-                cv[t][1] = cp[1];
+                const float2* cp = Cz + (int64_t)m * R.ka + 16 * min(t, T - 1) + 4 * g;
+                cv[t][0] = get4<COH>(cp);
+                cv[t][1] = get4<COH>(cp + 2);
             }
             floatx4 acc[3];
 #pragma unroll
@@ -740,12 +788,12 @@ __device__ __attribute__((always_inline)) inline void xcontract_columns(const ZM
 #pragma unroll
             for (int ct = 0; ct < 3; ++ct) acc[ct] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
             for (int t = 0; t < T; ++t) {
-                const floatx4* cp = reinterpret_cast<const floatx4*>(Cz + (int64_t)m * R.ka + 16 * t + 4 * g);
+                const float2* cp = Cz + (int64_t)m * R.ka + 16 * t + 4 * g;
                 floatx4 xv[3][2];
 #pragma unroll
                 for (int ct = 0; ct < 3; ++ct) xload(ct, t, xv[ct][0], xv[ct][1]);
                 bf16x8 ah, al;
-                xc_a_operand(cp[0], cp[1], o, ah, al);
+                xc_a_operand(get4<COH>(cp), get4<COH>(cp + 2), o, ah, al);
 #pragma unroll
                 for (int ct = 0; ct < 3; ++ct) {
                     bf16x8 bh, bl;
@@ -758,20 +806,25 @@ __device__ __attribute__((always_inline)) inline void xcontract_columns(const ZM
     }
 }
 
-template <int KMAX, bool VEC, int MODE>
-__global__ __launch_bounds__(256) void main_16(const ZMeta* __restrict__ zmeta, const RegionDev* __restrict__ regions,
-                                               const float2* __restrict__ Cs, const float2* __restrict__ X,
-                                               const uint16_t* __restrict__ Yfrag, int64_t ylo_delta, int nx,
-                                               int ny, int ytiles_per_split, float* __restrict__ out, int zoff) {
+// LDS of main_16: the workgroup's U_z columns while the fragments are built,
+// then the double-buffered output transpose.
+template <int KMAX>
+struct Main16Lds {
+    static constexpr int kTBytes = 2 * 16 * TROW * 4, kUBytes = KMAX / 2 * UPL * 8;
+    static constexpr int kXS = xt_staged<KMAX>(), kXBytes = 3 * kXS * 64 * (int)sizeof(XFrag);
+    static constexpr int kBytes = kTBytes > kUBytes + kXBytes ? kTBytes : kUBytes + kXBytes;
+};
+
+// One main-GEMM workgroup: column block bx, z-slice z, y split bz.
+template <int KMAX, bool VEC, int MODE, bool COH>
+__device__ inline void main_16_body(int bx, int z, int bz, const ZMeta* __restrict__ zmeta,
+                                    const RegionDev* __restrict__ regions, const float2* __restrict__ Cs,
+                                    const float2* __restrict__ X, const uint16_t* __restrict__ Yfrag,
+                                    int64_t ylo_delta, int nx, int ny, int ytiles_per_split, float* __restrict__ out,
+                                    unsigned char* lds) {
     typedef typename Op16<MODE>::vec vec;
     typedef typename Op16<MODE>::elem elem;
     constexpr int S = KMAX / 32;
-    // logical block (column block bx, z, y split) in XCD-contiguous order: the
-    // column blocks of one z and the z of one region share an XCD's L2 for
-    // C_z, X and the Y fragments
-    int bx, by, bz;
-    xcd_block3(bx, by, bz);
-    const int z = by + zoff;
     const ZMeta zm = zmeta[z];
     const int K32 = zm.kp >> 5;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -786,18 +839,12 @@ __global__ __launch_bounds__(256) void main_16(const ZMeta* __restrict__ zmeta, 
         zero_tiles<VEC>(oz, ncols, ny, yt0, yt1, (int64_t)bx * 256);
         return;
     }
-    // LDS: the workgroup's U_z columns while the fragments are built, then the
-    // double-buffered output transpose
-    constexpr int kTBytes = 2 * 16 * TROW * 4, kUBytes = KMAX / 2 * UPL * 8;
-    constexpr int kXS = xt_staged<KMAX>(), kXBytes = 3 * kXS * 64 * (int)sizeof(XFrag);
-    constexpr int kBytes = kTBytes > kUBytes + kXBytes ? kTBytes : kUBytes + kXBytes;
-    __shared__ __attribute__((aligned(16))) unsigned char lds[kBytes];
+    typedef Main16Lds<KMAX> L;
     float (*T)[16][TROW] = reinterpret_cast<float (*)[16][TROW]>(lds);
     float2 (*Ul)[UPL] = reinterpret_cast<float2 (*)[UPL]>(lds);
-    XFrag* xs = reinterpret_cast<XFrag*>(lds + kUBytes);
-    xcontract_columns<kXS>(zm, regions[zm.region], Cs, X, nx, bx, Ul, xs);
+    XFrag* xs = reinterpret_cast<XFrag*>(lds + L::kUBytes);
+    xcontract_columns<L::kXS, COH>(zm, regions[zm.region], Cs, X, nx, bx, Ul, xs);
     __syncthreads();
-
     // A fragment: Bp[k][n] from U_z[b = k >> 1][n >> 1]; rows past my are zero
     vec ahi[4][S], alo[4][S];
 #pragma unroll
@@ -865,6 +912,22 @@ __global__ __launch_bounds__(256) void main_16(const ZMeta* __restrict__ zmeta, 
         yload(min(yt + 3, yt1 - 1), h1, l1);
     }
     if (yt < yt1) tile(yt, h0, l0);
+}
+
+
+template <int KMAX, bool VEC, int MODE>
+__global__ __launch_bounds__(256) void main_16(const ZMeta* __restrict__ zmeta, const RegionDev* __restrict__ regions,
+                                               const float2* __restrict__ Cs, const float2* __restrict__ X,
+                                               const uint16_t* __restrict__ Yfrag, int64_t ylo_delta, int nx,
+                                               int ny, int ytiles_per_split, float* __restrict__ out, int zoff) {
+    // logical block (column block bx, z, y split) in XCD-contiguous order: the
+    // column blocks of one z and the z of one region share an XCD's L2 for
+    // C_z, X and the Y fragments
+    int bx, by, bz;
+    xcd_block3(bx, by, bz);
+    __shared__ __attribute__((aligned(16))) unsigned char lds[Main16Lds<KMAX>::kBytes];
+    main_16_body<KMAX, VEC, MODE, false>(bx, by + zoff, bz, zmeta, regions, Cs, X, Yfrag, ylo_delta, nx, ny,
+                                         ytiles_per_split, out, lds);
 }
 
 template <int KMAX, bool VEC>
@@ -1082,7 +1145,7 @@ int shift_bfp_exponents(const uint8_t* src, uint8_t* dst, int64_t erow, int64_t 
 // Partial sums of a mode-contraction item into its slot of C: lane holds
 // D[i = 4 g + r][m0 + 16 nt + j]: rows 4g..4g+3 are (z, re), (z, im),
 // (z + 1, re), (z + 1, im) with z = 8 mt + 2 g.
-template <int MT>
+template <int MT, bool COH>
 __device__ inline void modal_store(const floatx4 (&acc)[4][MT], const ModalItem& I, const ZMeta* __restrict__ zmeta,
                                    int64_t c_elems, float2* __restrict__ C, int mcols) {
     const int lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4;
@@ -1096,12 +1159,12 @@ __device__ inline void modal_store(const floatx4 (&acc)[4][MT], const ModalItem&
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt) {
                 const int m = I.m0 + 16 * nt + j;
-                if (m < mcols) o[m] = make_float2(acc[nt][mt][2 * h], acc[nt][mt][2 * h + 1]);
+                if (m < mcols) put2<COH>(o + m, make_float2(acc[nt][mt][2 * h], acc[nt][mt][2 * h + 1]));
             }
         }
 }
 
-template <int MODE, int MT>
+template <int MODE, int MT, bool COH>
 __device__ inline void modal_body(const ModalItem& I, const RegionDev& R, const ZMeta* __restrict__ zmeta,
                                   const uint32_t* __restrict__ zf, int64_t c_elems, float2* __restrict__ C) {
     const int lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4;
@@ -1202,7 +1265,7 @@ __device__ inline void modal_body(const ModalItem& I, const RegionDev& R, const 
         step(a1, z1);
         if (++t == t1) break;
     }
-    modal_store<MT>(acc, I, zmeta, c_elems, C, mcols);
+    modal_store<MT, COH>(acc, I, zmeta, c_elems, C, mcols);
 }
 
 // Runs of 64 z (MT = 8 row tiles) with the 16-bit modes.  All four waves of
@@ -1218,13 +1281,19 @@ __device__ inline void modal_body(const ModalItem& I, const RegionDev& R, const 
 // (each step's raw records are decoded once into hi/lo operands for all four
 // column tiles, and their registers take the load of the step after next).
 template <int MODE>
+struct ModalWgLds {  // the double-buffered phase fragments of a step
+    static constexpr int MT = 8, NH = MODE == 1 ? 2 : 1;
+    static constexpr int kBytes = 2 * MT * NH * 64 * (int)sizeof(uint32x4);
+};
+
+template <int MODE, bool COH>
 __device__ inline void modal_body_wg(const ModalItem& I, const ModalItem& L, const RegionDev& R,
                                      const ZMeta* __restrict__ zmeta, const uint32_t* __restrict__ zf,
-                                     int64_t c_elems, float2* __restrict__ C) {
+                                     int64_t c_elems, float2* __restrict__ C, unsigned char* lds) {
     static_assert(MODE != 3, "16-bit modes");
     constexpr int MT = 8, NH = MODE == 1 ? 2 : 1;  // fragment halves: high (and low)
     typedef typename Op16<MODE>::vec vec;
-    __shared__ uint32x4 zs[2][MT][NH][64];
+    uint32x4 (*zs)[MT][NH][64] = reinterpret_cast<uint32x4 (*)[MT][NH][64]>(lds);
     const int lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4;
     const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const int mcols = 3 * R.cells;
@@ -1312,7 +1381,7 @@ __device__ inline void modal_body_wg(const ModalItem& I, const ModalItem& L, con
         step(a1, t, 1);
         if (++t == t1) break;
     }
-    modal_store<MT>(acc, I, zmeta, c_elems, C, mcols);
+    modal_store<MT, COH>(acc, I, zmeta, c_elems, C, mcols);
 }
 
 template <int MODE, int MT>
@@ -1326,7 +1395,7 @@ __global__ __launch_bounds__(256) void modal_kernel(const ModalItem* __restrict_
     const int it = __builtin_amdgcn_readfirstlane(lb * 4 + (int)(threadIdx.x >> 6));
     if (it >= nitems) return;
     const ModalItem I = items[it];
-    modal_body<MODE, MT>(I, regions[I.region], zmeta, zf, c_elems, C);
+    modal_body<MODE, MT, false>(I, regions[I.region], zmeta, zf, c_elems, C);
 }
 
 // The 64-z class of the 16-bit modes (modal_body_wg), two waves per SIMD.
@@ -1342,7 +1411,8 @@ __global__ __launch_bounds__(256, 2) void modal_wg_kernel(const ModalItem* __res
     if ((int)blockIdx.x * 4 + 3 >= nitems) return;
     const int it = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
     const ModalItem I = items[it];
-    modal_body_wg<MODE>(I, items[blockIdx.x * 4], regions[I.region], zmeta, zf, c_elems, C);
+    __shared__ __attribute__((aligned(16))) unsigned char lds[ModalWgLds<MODE>::kBytes];
+    modal_body_wg<MODE, false>(I, items[blockIdx.x * 4], regions[I.region], zmeta, zf, c_elems, C, lds);
 }
 
 template <int MODE>
