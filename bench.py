@@ -50,13 +50,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def time_steps(ex, out, stream, steps, warmup):
+def time_steps(ex, out, stream, steps, warmup, single="auto"):
     """Device ms per step over `steps` back-to-back runs, launched two ways:
     eager (scw_exec_run: the stage kernels enqueued directly) and replayed from
-    a captured hipGraph.  Returns (ms of the faster, wall ms of it, which,
-    {way: ms}).  On this runtime a graph replay leaves a few microseconds
-    idle between consecutive replays that eager launches do not."""
-    from scwatter_amd import hip
+    a captured hipGraph; with `single` "auto" or "on" (16-bit GEMM dtypes),
+    the same two ways again in the single-launch form (one persistent kernel
+    per run, scw_exec_set_persistent).  Returns (ms of the fastest, wall ms of
+    it, which, {way: ms}) and leaves the exec in the form of the fastest way.
+    On this runtime a graph replay leaves a few microseconds idle between
+    consecutive replays that eager launches do not."""
+    from scwatter_amd import ScwError, hip
     e0, e1 = hip.Event(), hip.Event()
 
     def timed(launch):
@@ -73,10 +76,31 @@ def time_steps(ex, out, stream, steps, warmup):
         hip.synchronize()
         return e0.elapsed_ms(e1) / steps, (time.perf_counter() - t0) * 1e3 / steps
 
-    res = {"eager": timed(lambda: ex.run(out.ptr, stream.handle))}
-    ex.graph_capture(out.ptr, stream.handle)
-    res["graph"] = timed(lambda: ex.graph_launch(stream.handle))
+    res = {}
+    has_single = False
+    if single != "off":
+        try:
+            ex.set_persistent(True)
+            has_single = True
+        except ScwError:  # FP32 or the direct form: stage-by-stage only
+            pass
+    if single == "off" or single == "auto" or not has_single:
+        if has_single:
+            ex.set_persistent(False)
+        res["eager"] = timed(lambda: ex.run(out.ptr, stream.handle))
+        ex.graph_capture(out.ptr, stream.handle)
+        res["graph"] = timed(lambda: ex.graph_launch(stream.handle))
+    if has_single:
+        ex.set_persistent(True)
+        res["single_eager"] = timed(lambda: ex.run(out.ptr, stream.handle))
+        ex.graph_capture(out.ptr, stream.handle)
+        res["single_graph"] = timed(lambda: ex.graph_launch(stream.handle))
+        ex.check(stream.handle)
     way = min(res, key=lambda k: res[k][0])
+    if has_single and not way.startswith("single"):
+        ex.set_persistent(False)
+        ex.run(out.ptr, stream.handle)  # the output of the chosen form
+        stream.synchronize()
     return res[way][0], res[way][1], way, {k: round(v[0], 4) for k, v in res.items()}
 
 
@@ -230,6 +254,9 @@ def main():
     ap.add_argument("--physical", action="store_true", help="tile sample (c2): eigen-tables from the host sample "
                     "solver instead of seeded synthetic ones")
     ap.add_argument("--no-drop-in", action="store_true", help="leave out the drop-in latency measurement")
+    ap.add_argument("--single-launch", default="auto", choices=("auto", "on", "off"),
+                    help="16-bit GEMM dtypes: also time the single-launch form and keep the faster (auto), "
+                         "time only it (on), or only the stage-by-stage launches (off)")
     ap.add_argument("--retune", action="store_true", help="time the launch-shape candidates even if the "
                     "tuning table has an entry for the run")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -287,7 +314,7 @@ def main():
         t0 = time.perf_counter()
         ex = Exec(plan, xs, ys, zs, form="gemm", dtype=dt)
         info = ex.info()
-        ms, wall_ms, way, by_way = time_steps(ex, out, stream, args.steps, args.warmup)
+        ms, wall_ms, way, by_way = time_steps(ex, out, stream, args.steps, args.warmup, args.single_launch)
         samp = sample_device(out, shape, pidx)
         stages = {}
         sms = time_stages(ex, out, stream, max(3, args.steps // 2))
@@ -496,7 +523,7 @@ def bench_batch(args):
             tuned = {"coef_items": best[0], "main_blocks": best[1], "source": "tuned",
                      "candidates_ms": {f"{a}/{b}": round(v, 4) for (a, b), v in cand.items()}}
         info = bx.info()
-        ms, wall_ms, way, by_way = time_steps(bx, out, stream, args.steps, args.warmup)
+        ms, wall_ms, way, by_way = time_steps(bx, out, stream, args.steps, args.warmup, args.single_launch)
         sms = time_stages(bx, out, stream, 3)
         stages = {}
         for st, name in ((1, "coef"), (2, "xcontract"), (3, "main")):

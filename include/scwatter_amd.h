@@ -58,7 +58,8 @@ enum scw_status {
     SCW_ERR_DEVICE = -3,      /* HIP runtime error / no device                 */
     SCW_ERR_IO = -4,          /* file missing, unreadable or truncated         */
     SCW_ERR_FORMAT = -5,      /* .cw content invalid                           */
-    SCW_ERR_UNSUPPORTED = -6  /* form/dtype not available for this structure  */
+    SCW_ERR_UNSUPPORTED = -6, /* form/dtype not available for this structure  */
+    SCW_ERR_TIMEOUT = -7      /* a single-launch run's dependency wait expired */
 };
 
 enum scw_form {
@@ -148,6 +149,28 @@ int scw_batch_exec_create(int n, scw_plan* const* plans, const double* xs, size_
  * and graph launch of the exec returns SCW_ERR_ARG until a later call
  * succeeds; the exec can always be destroyed. */
 int scw_exec_set_launch(scw_exec* exec, int64_t coef_items, int64_t main_blocks);
+/* Single-launch form of a GEMM-form exec with a 16-bit dtype (no
+ * counterpart in the reference, whose gpu_cw_evaluate launches one kernel per
+ * plane, gpuEvaluator.cu:149-157): with on != 0, every later run (and graph
+ * capture) is ONE persistent kernel launch that takes the coefficient,
+ * slot-sum and main-GEMM work units of the run from a device queue in
+ * dependency order; a unit waits for the units it reads on device counters,
+ * each wait at most wait_ms (0 = 10000 ms).  workgroups = 0 launches the
+ * resident capacity (workgroups per CU x CUs); a larger request is clamped
+ * to it.  Results are bit-identical to the stage-by-stage runs.  An expired
+ * wait does not stop the launch (it still ends) but leaves its output
+ * invalid and sets an error word that scw_exec_check reports.  on = 0
+ * returns to stage-by-stage launches.  Drops a captured graph; the form
+ * survives scw_exec_set_launch.  SCW_ERR_UNSUPPORTED for FP32 or the direct
+ * form.  Runs of one exec must not overlap (they share its counters). */
+int scw_exec_set_persistent(scw_exec* exec, int on, int64_t workgroups, double wait_ms);
+/* on: whether the single-launch form is active; units / workgroups: its work
+ * units and launch size (0 when off).  Pointers may be NULL. */
+int scw_exec_persistent_info(const scw_exec* exec, int* on, int64_t* units, int64_t* workgroups);
+/* Waits for `stream`, then reports (and clears) an expired wait of the
+ * single-launch runs since the last check: SCW_ERR_TIMEOUT, else SCW_OK
+ * (always SCW_OK for stage-by-stage execs). */
+int scw_exec_check(scw_exec* exec, void* stream);
 /* Capture one run into a hipGraph and replay it with scw_exec_graph_launch. */
 int scw_exec_graph_capture(scw_exec* exec, void* out_device, void* stream);
 int scw_exec_graph_launch(scw_exec* exec, void* stream);
@@ -218,6 +241,11 @@ int scw_post_color(const void* scalar_device, size_t n, const float* lowhigh_dev
  * (k < 0: off), so that tests can drive the allocation-failure paths without
  * exhausting the device.  Process-wide; not for production use. */
 void scw_test_fail_alloc_after(int64_t k);
+/* With on != 0, the single-launch form of `exec` counts one slot-sum unit too
+ * many for z-slice 0, so its main units' wait can never be met and must
+ * expire (a host-side condition: every kernel still ends normally); on = 0
+ * restores the counts.  Not for production use. */
+int scw_test_exec_stall(scw_exec* exec, int on);
 
 /* ---- .cw files (CoupledWaveStructure.h:7-134) --------------------------- */
 int scw_cw_load(const char* path, scw_cw** cw);

@@ -15,7 +15,7 @@ FORM_NAMES = {v: k for k, v in FORMS.items()}
 DTYPES = {"fp32": 0, "bf16": 1, "bf16x3": 2, "fp16": 3}
 DTYPE_NAMES = {v: k for k, v in DTYPES.items()}
 STATUS = {0: "SCW_OK", -1: "SCW_ERR_ARG", -2: "SCW_ERR_NOMEM", -3: "SCW_ERR_DEVICE", -4: "SCW_ERR_IO",
-          -5: "SCW_ERR_FORMAT", -6: "SCW_ERR_UNSUPPORTED"}
+          -5: "SCW_ERR_FORMAT", -6: "SCW_ERR_UNSUPPORTED", -7: "SCW_ERR_TIMEOUT"}
 
 # Every extern "C" symbol declared in include/scwatter_amd.h.
 EXPORTS = (
@@ -23,7 +23,8 @@ EXPORTS = (
     "scw_plan_create", "scw_plan_destroy", "scw_plan_layers", "scw_plan_region_info", "scw_plan_device_bytes",
     "scw_exec_create", "scw_exec_run", "scw_exec_run_stage", "scw_exec_run_host", "scw_exec_destroy", "scw_exec_info",
     "scw_exec_stage_info", "scw_batch_exec_create", "scw_exec_set_launch",
-    "scw_exec_graph_capture", "scw_exec_graph_launch",
+    "scw_exec_graph_capture", "scw_exec_graph_launch", "scw_exec_set_persistent", "scw_exec_persistent_info",
+    "scw_exec_check", "scw_test_exec_stall",
     "scw_cw_load", "scw_cw_save", "scw_cw_create", "scw_cw_destroy", "scw_cw_set_incident", "scw_cw_add_layer",
     "scw_cw_counts", "scw_cw_layer", "scw_cw_unpack",
     "scw_post_scalar", "scw_post_minmax", "scw_post_color", "scw_test_fail_alloc_after", "scw_eval_planes",
@@ -80,6 +81,10 @@ def lib():
         "scw_exec_set_launch": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64]),
         "scw_exec_graph_capture": (ctypes.c_int, [_vp, _vp, _vp]),
         "scw_exec_graph_launch": (ctypes.c_int, [_vp, _vp]),
+        "scw_exec_set_persistent": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int64, ctypes.c_double]),
+        "scw_exec_persistent_info": (ctypes.c_int, [_vp, _intp, _i64p, _i64p]),
+        "scw_exec_check": (ctypes.c_int, [_vp, _vp]),
+        "scw_test_exec_stall": (ctypes.c_int, [_vp, ctypes.c_int]),
         "scw_cw_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
         "scw_cw_save": (ctypes.c_int, [_vp, ctypes.c_char_p]),
         "scw_cw_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),

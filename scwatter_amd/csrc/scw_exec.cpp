@@ -98,6 +98,127 @@ static inline bool region_ok(const cd v) {
     return std::isfinite(v.real()) && std::isfinite(v.imag()) && std::abs(v) <= 1e30;
 }
 
+// Kernel launches of a stage-by-stage GEMM-form run.
+static int stage_launches(const scw_exec* ex) {
+    int n = ex->items.empty() ? 0 : 1;
+    for (int k = 0; k < 4; ++k) n += ex->mclass_off[k + 1] > ex->mclass_off[k] ? 1 : 0;
+    const bool xstage = ex->dtype == SCW_DTYPE_FP32 ? ex->my_max > 0 : ex->ncblk > 0;  // u_mfma / slot sum
+    return n + (xstage ? 1 : 0) + 1;
+}
+
+// Work units of the single-launch kernel (kinds and fields as fused16 in
+// scw_gemm.hip reads them), the counts its waits compare against (per z: the
+// wave items covering z, then the slot-sum units of z), its zeroed control
+// words, its grid (the resident capacity, or fewer) and its wait limit.
+//
+// Order: the coefficient units sorted by their first z; the slot-sum units
+// of z placed `lag` coefficient units after the last one covering z, and the
+// main units of z `lag` units after that, so that by the time a workgroup
+// takes a unit, what it waits for was taken about `lag` units earlier and is
+// most likely done (a smaller lag makes waiting workgroups pile up behind
+// unfinished items; an unbounded one is the stage order).  Every unit comes
+// after the units it depends on.
+static int build_units(scw_exec* ex) {
+    enum { kCoef = 0, kModal1 = 1, kModalWg = 4, kCsum = 5, kMain = 6, kCtlWords = 4 };
+    const int64_t nz = ex->nz;
+    const int64_t gx = (6 * ex->nx + 255) / 256;
+    const int tiles = (int)std::max<int64_t>(1, std::min<int64_t>(255, ex->main_tiles > 0 ? ex->main_tiles
+                                                                                       : tunable("SCW_SINGLE_MAIN_TILES", 2)));
+    const int cgroup = (int)std::max<int64_t>(1, tunable("SCW_SINGLE_CSUM_BLOCKS", 8));
+    const int64_t n = ((int64_t)ex->items.size() + 3) / 4 + ((int64_t)ex->mitems.size() + 12) / 4 +
+                      (int64_t)ex->hcblk.size() + gx * nz * ex->ysplit;
+    if (n > (int64_t)INT32_MAX - (1 << 20))
+        return fail(SCW_ERR_UNSUPPORTED, "single-launch form: %lld work units exceed 32-bit indices", (long long)n);
+    std::vector<int> need((size_t)(2 * nz), 0);
+    // coefficient units with their first z
+    std::vector<std::pair<int, int4>> cu;
+    const int nitems = (int)ex->items.size();
+    for (int i = 0; i < nitems; i += 4) cu.push_back({ex->items[i].z0, make_int4(kCoef, i, nitems, 0)});
+    for (const CoefItem& it : ex->items)
+        for (int z = it.z0; z < it.z0 + it.nz; ++z) ++need[z];
+    for (int k = 0; k < 4; ++k) {
+        const int off = (int)ex->mclass_off[k], end = (int)ex->mclass_off[k + 1];
+        if (k == 3 && (end - off) % 4 != 0 && ex->dtype != SCW_DTYPE_FP32)
+            return fail(SCW_ERR_ARG, "internal: 64-z mode items not in whole workgroups");
+        for (int i = off; i < end; i += 4) {
+            int z0 = ex->mitems[i].z0;
+            for (int w = 1; w < 4 && i + w < end; ++w) z0 = std::min(z0, ex->mitems[i + w].z0);
+            cu.push_back({z0, make_int4(k < 3 ? kModal1 + k : kModalWg, i, end, 0)});
+        }
+    }
+    for (const ModalItem& it : ex->mitems)
+        for (int z = it.z0; z < it.z0 + it.nz; ++z) ++need[z];
+    std::stable_sort(cu.begin(), cu.end(), [](const std::pair<int, int4>& a, const std::pair<int, int4>& b) {
+        return a.first < b.first;
+    });
+    // position of the last coefficient unit covering each z
+    std::vector<int64_t> last((size_t)nz, -1);
+    for (size_t p = 0; p < cu.size(); ++p) {
+        const int4 U = cu[p].second;
+        const int end = U.z;
+        for (int w = 0; w < 4 && U.y + w < end; ++w) {
+            int z0, zn;
+            if (U.x == kCoef) {
+                z0 = ex->items[U.y + w].z0;
+                zn = ex->items[U.y + w].nz;
+            } else {
+                z0 = ex->mitems[U.y + w].z0;
+                zn = ex->mitems[U.y + w].nz;
+            }
+            for (int z = z0; z < z0 + zn; ++z) last[z] = std::max<int64_t>(last[z], (int64_t)p);
+        }
+    }
+    const int64_t lag = std::max<int64_t>(0, ex->order_lag >= 0 ? ex->order_lag : tunable("SCW_SINGLE_LAG", 1 << 30));
+    // (key, unit): key = 4 x position in the coefficient stream + kind rank
+    std::vector<std::pair<int64_t, int4>> all;
+    all.reserve((size_t)n);
+    for (size_t p = 0; p < cu.size(); ++p) all.push_back({4 * (int64_t)p, cu[p].second});
+    std::vector<int64_t> ckey((size_t)nz, -4);
+    for (size_t c = 0; c < ex->hcblk.size();) {
+        const int z = ex->hcblk[c].x;
+        size_t e = c;
+        while (e < ex->hcblk.size() && ex->hcblk[e].x == z && (int)(e - c) < cgroup) ++e;
+        const int64_t pos = last[z] < 0 ? -1 : last[z] + lag;
+        all.push_back({4 * pos + 1, make_int4(kCsum, (int)c, (int)(e - c), 0)});
+        ckey[z] = std::max(ckey[z], pos);
+        ++need[nz + z];
+        c = e;
+    }
+    for (int z = 0; z < nz; ++z) {
+        const int64_t pos = ckey[z] < -1 ? -1 : ckey[z] + lag;
+        for (int bz = 0; bz < ex->ysplit; ++bz)
+            for (int bx = 0; bx < gx; bx += tiles)
+                all.push_back({4 * pos + 2, make_int4(kMain | ((int)std::min<int64_t>(tiles, gx - bx) << 8), bx, z, bz)});
+    }
+    std::stable_sort(all.begin(), all.end(), [](const std::pair<int64_t, int4>& a, const std::pair<int64_t, int4>& b) {
+        return a.first < b.first;
+    });
+    std::vector<int4> units(all.size());
+    for (size_t k = 0; k < all.size(); ++k) units[k] = all[k].second;
+    if (ex->stall) ++need[nz];  // the main units of z-slice 0 wait for a slot sum that never comes
+    int st;
+    if ((st = ex->d_units.upload(units.data(), units.size() * sizeof(int4))) != SCW_OK) return st;
+    if ((st = ex->d_need.upload(need.data(), need.size() * sizeof(int))) != SCW_OK) return st;
+    const size_t ctl = (size_t)(kCtlWords + 2 * nz) * sizeof(int);
+    if ((st = ex->d_ctl.alloc(ctl)) != SCW_OK) return st;
+    SCW_HIP(hipMemset(ex->d_ctl.p, 0, ctl));
+    ex->nunits = (int64_t)units.size();
+    int per_cu = 0, cus = 0, khz = 0, dev = 0;
+    if ((st = fused_launch(ex, nullptr, nullptr, &per_cu)) != SCW_OK) return st;
+    SCW_HIP(hipGetDevice(&dev));
+    SCW_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    SCW_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    const int64_t cap = (int64_t)per_cu * cus;
+    if (cap < 1) return fail(SCW_ERR_DEVICE, "single-launch kernel: no workgroup fits a CU");
+    // at most the resident capacity: the first unit of workgroup b is unit b,
+    // which must not wait behind workgroups that cannot start
+    ex->pgrid = std::min<int64_t>(ex->nunits, ex->tune_grid > 0 ? std::min(ex->tune_grid, cap) : cap);
+    ex->wait_ticks = (uint64_t)std::max(1.0, ex->wait_ms * (double)std::max(khz, 1));
+    ex->wait_polls = (uint64_t)std::max(1000.0, ex->wait_ms * 1.0e5);
+    ex->launches = 1;
+    return SCW_OK;
+}
+
 // Builds every GEMM-form table of a run over the z entries ze (one per output
 // z-slice, in output order).  Regions are identified by (plan, region) pairs,
 // so a run may mix structures (batches).
@@ -358,7 +479,9 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
         m.yfrag_off = yfrag_off[v];
         m.zrow_off = (int64_t)Z.size() / 2;
         m.c_off = c_off;
-        u_off += is16 ? 3LL * R.my * rd[v].ka : (int64_t)R.my * nx * 3;
+        // 16-bit dtypes: every Cs_z on its own 256-byte lines (the
+        // single-launch kernel reads them through the caches once final)
+        u_off += is16 ? (3LL * R.my * rd[v].ka + 31) / 32 * 32 : (int64_t)R.my * nx * 3;
         c_off += (int64_t)R.mx * R.my * 3;
         for (int c = 0; c < R.mc; ++c) {
             const cd val = std::exp(cd(0, 1) * R.kz[c] * (ze[z].z - R.zref));
@@ -566,7 +689,7 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
     ex->stage_bytes[3] = b_main;
     ex->stage_flops[0] = ex->flops = f_coef + f_u + f_main;
     ex->stage_bytes[0] = ex->bytes = b_coef + b_u + b_main;
-    ex->launches = (ex->items.empty() ? 0 : 1) + (int)((ex->mclass_off[1] > ex->mclass_off[0]) + (ex->mclass_off[2] > ex->mclass_off[1]) + (ex->mclass_off[3] > ex->mclass_off[2]) + (ex->mclass_off[4] > ex->mclass_off[3])) + (ex->my_max > 0 ? 1 : 0) + 1;
+    ex->launches = stage_launches(ex);
 
     const int gx = (int)((6LL * nx + 255) / 256);
     // Each wave keeps its A-operand fragment (from U_z) in registers for all
@@ -601,6 +724,8 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
         ex->ylo_delta = (int64_t)yhi.size();
         if ((st = ex->d_Yfrag.upload(both.data(), both.size() * 2)) != SCW_OK) return st;
     }
+    ex->hcblk.swap(cblk);
+    if (ex->persistent) return build_units(ex);
     return SCW_OK;
 }
 
@@ -617,8 +742,11 @@ static std::vector<ZEntry> entries(const scw_exec* ex) {
 // Drops the GEMM-form tables and work lists (before a fall back to the direct form).
 static void release_gemm(scw_exec* ex) {
     for (DevBuf* b : {&ex->d_items, &ex->d_mitems, &ex->d_regions, &ex->d_zmeta, &ex->d_Z, &ex->d_Zf, &ex->d_X, &ex->d_C,
-                      &ex->d_U, &ex->d_Yfrag, &ex->d_bfpx, &ex->d_cblk})
+                      &ex->d_U, &ex->d_Yfrag, &ex->d_bfpx, &ex->d_cblk, &ex->d_units, &ex->d_need, &ex->d_ctl})
         b->release();
+    ex->persistent = false;
+    ex->nunits = ex->pgrid = 0;
+    ex->hcblk.clear();
     ex->items.clear();
     ex->mitems.clear();
     for (int k = 0; k <= 4; ++k) ex->mclass_off[k] = 0;
@@ -633,7 +761,8 @@ static void exec_free(scw_exec* ex) {
     if (ex->graph) (void)hipGraphDestroy(ex->graph);
     if (ex->graph_stream) (void)hipStreamDestroy(ex->graph_stream);
     for (DevBuf* b : {&ex->d_dx, &ex->d_dy, &ex->d_z, &ex->d_zslab, &ex->d_items, &ex->d_mitems, &ex->d_regions, &ex->d_zmeta,
-                      &ex->d_Z, &ex->d_Zf, &ex->d_X, &ex->d_C, &ex->d_U, &ex->d_Yfrag, &ex->d_bfpx, &ex->d_cblk, &ex->d_out})
+                      &ex->d_Z, &ex->d_Zf, &ex->d_X, &ex->d_C, &ex->d_U, &ex->d_Yfrag, &ex->d_bfpx, &ex->d_cblk,
+                      &ex->d_units, &ex->d_need, &ex->d_ctl, &ex->d_out})
         b->release();
     delete ex;
 }
@@ -675,6 +804,7 @@ int exec_init(scw_exec* ex, scw_plan* plan, const double* xs, size_t nx, const d
     }
     ex->plan = plan;
     ex->batch.clear();
+    if (dtype == SCW_DTYPE_FP32) ex->persistent = false;  // the single-launch form takes 16-bit dtypes
     ex->nx = (int64_t)nx;
     ex->ny = (int64_t)ny;
     ex->nz = (int64_t)nz;
@@ -853,6 +983,88 @@ int scw_exec_set_launch(scw_exec* ex, int64_t coef_items, int64_t main_blocks) {
     }
     if (st == SCW_OK) ex->invalid = false;
     return st;
+}
+
+static int refuse_invalid(const scw_exec* ex);
+
+static void drop_graph(scw_exec* ex) {
+    if (ex->graph_exec) {
+        (void)hipGraphExecDestroy(ex->graph_exec);
+        ex->graph_exec = nullptr;
+    }
+    if (ex->graph) {
+        (void)hipGraphDestroy(ex->graph);
+        ex->graph = nullptr;
+    }
+}
+
+int scw_exec_set_persistent(scw_exec* ex, int on, int64_t workgroups, double wait_ms) {
+    if (!ex) return fail(SCW_ERR_ARG, "exec is NULL");
+    if (workgroups < 0) return fail(SCW_ERR_ARG, "negative workgroup count");
+    if (!std::isfinite(wait_ms) || wait_ms < 0 || wait_ms > 3.6e6)
+        return fail(SCW_ERR_ARG, "wait limit must be in [0, 3.6e6] ms (0 = default)");
+    if (on && (ex->form != SCW_FORM_GEMM || ex->dtype == SCW_DTYPE_FP32))
+        return fail(SCW_ERR_UNSUPPORTED, "single-launch form: GEMM form with a 16-bit dtype only");
+    if (ex->invalid) return refuse_invalid(ex);
+    DeviceGuard g(ex->plan->device);
+    if (g.st != SCW_OK) return g.st;
+    drop_graph(ex);
+    for (DevBuf* b : {&ex->d_units, &ex->d_need, &ex->d_ctl}) b->release();
+    ex->persistent = false;
+    ex->nunits = ex->pgrid = 0;
+    ex->launches = ex->form == SCW_FORM_GEMM ? stage_launches(ex) : ex->launches;
+    if (!on) return SCW_OK;
+    ex->tune_grid = workgroups;
+    ex->wait_ms = wait_ms > 0 ? wait_ms : 10000.0;
+    ex->persistent = true;
+    int st;
+    try {
+        st = build_units(ex);
+    } catch (const std::bad_alloc&) {
+        st = fail(SCW_ERR_NOMEM, "host allocation failed while building work units");
+    }
+    if (st != SCW_OK) {
+        // the stage-by-stage tables are untouched: the exec stays usable in that form
+        for (DevBuf* b : {&ex->d_units, &ex->d_need, &ex->d_ctl}) b->release();
+        ex->persistent = false;
+        ex->nunits = ex->pgrid = 0;
+        ex->launches = stage_launches(ex);
+    }
+    return st;
+}
+
+int scw_exec_persistent_info(const scw_exec* ex, int* on, int64_t* units, int64_t* workgroups) {
+    if (!ex) return fail(SCW_ERR_ARG, "exec is NULL");
+    if (on) *on = ex->persistent ? 1 : 0;
+    if (units) *units = ex->nunits;
+    if (workgroups) *workgroups = ex->pgrid;
+    return SCW_OK;
+}
+
+int scw_exec_check(scw_exec* ex, void* stream) {
+    if (!ex) return fail(SCW_ERR_ARG, "exec is NULL");
+    if (!ex->persistent || !ex->d_ctl.p) return SCW_OK;
+    DeviceGuard g(ex->plan->device);
+    if (g.st != SCW_OK) return g.st;
+    SCW_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    int* err = ex->d_ctl.as<int>() + 2;  // the error word (kCtlErr)
+    int v = 0;
+    SCW_HIP(hipMemcpy(&v, err, sizeof(int), hipMemcpyDeviceToHost));
+    if (v == 0) return SCW_OK;
+    const int zero = 0;
+    SCW_HIP(hipMemcpy(err, &zero, sizeof(int), hipMemcpyHostToDevice));
+    return fail(SCW_ERR_TIMEOUT, "single-launch run: a dependency wait exceeded %.0f ms; its output is invalid",
+                ex->wait_ms);
+}
+
+int scw_test_exec_stall(scw_exec* ex, int on) {
+    if (!ex) return fail(SCW_ERR_ARG, "exec is NULL");
+    ex->stall = on != 0;
+    if (!ex->persistent) return SCW_OK;
+    DeviceGuard g(ex->plan->device);
+    if (g.st != SCW_OK) return g.st;
+    drop_graph(ex);
+    return build_units(ex);
 }
 
 static int refuse_invalid(const scw_exec* ex) {

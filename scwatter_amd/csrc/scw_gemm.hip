@@ -61,6 +61,16 @@ __device__ inline uint32x4 stream_load(const uint32x4* p) { return __builtin_non
 typedef const __attribute__((address_space(1))) uint32_t* global_uint32_ptr;
 __device__ inline uint32_t stream_load(const uint32_t* p) { return __builtin_nontemporal_load((global_uint32_ptr)(p)); }
 
+// threadIdx.x through an empty volatile asm: the compiler cannot hoist what is
+// computed from it out of a loop.  In the single-launch kernel, whose unit loop
+// runs every body, lane-derived addresses hoisted out of that loop stay live
+// across all bodies and push them into scratch spills.
+__device__ inline int tid_x() {
+    int t = (int)threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
 // Ordinary load through a pointer read from a device struct (the region
 // tables).  The compiler cannot infer the address space of such a pointer and
 // emits a FLAT load, which counts against both the vector-memory and the LDS
@@ -114,9 +124,10 @@ __device__ inline floatx4 get4(const float2* p) {
         return *reinterpret_cast<const floatx4*>(p);
     }
 }
-// Waits until this thread's stores have completed (vmcnt 0): with sc1
-// stores, they are then visible to every XCD.
-__device__ inline void stores_done() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+// Waits until this wave's stores have completed (vmcnt 0): with sc1 stores,
+// they are then visible to every XCD.  The memory clobber keeps the compiler
+// from moving the stores past it.
+__device__ inline void stores_done() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
 // ---------------------------------------------------------------------------
 // C_z[a,b,comp] partial sums over a term slice, for up to 8 z of one region.
@@ -224,7 +235,7 @@ template <bool COH>
 __device__ inline void coef_body(const CoefItem& I, const RegionDev* __restrict__ regions,
                                  const ZMeta* __restrict__ zmeta, const float2* __restrict__ Z, int64_t c_elems,
                                  float2* __restrict__ C) {
-    const int lane = threadIdx.x & 63;
+    const int lane = tid_x() & 63;
     const RegionDev R = regions[I.region];
     const int cell = I.cell0 + lane;
     const bool active = cell < R.cells;
@@ -277,7 +288,7 @@ __global__ __launch_bounds__(256) void coef_kernel(const CoefItem* __restrict__ 
                                                    const RegionDev* __restrict__ regions,
                                                    const ZMeta* __restrict__ zmeta, const float2* __restrict__ Z,
                                                    int64_t c_elems, float2* __restrict__ C) {
-    const int it = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+    const int it = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (tid_x() >> 6)));
     if (it >= nitems) return;
     coef_body<false>(items[it], regions, zmeta, Z, c_elems, C);
 }
@@ -310,7 +321,7 @@ __global__ __launch_bounds__(256) void u_mfma(const ZMeta* __restrict__ zmeta, c
     const int mx = R.mx, T = (mx + 15) >> 4, ka = 16 * T;
     // C_z[a][b0 + r / 3][r % 3], slots added in order (16 loads in flight);
     // zero past mx and past my.
-    for (int e = threadIdx.x; e < 3 * UB * ka; e += 256) {
+    for (int e = tid_x(); e < 3 * UB * ka; e += 256) {
         const int r = e / ka, a = e - r * ka;
         const int b = b0 + r / 3;
         float2 sum = make_float2(0.0f, 0.0f);
@@ -332,7 +343,7 @@ __global__ __launch_bounds__(256) void u_mfma(const ZMeta* __restrict__ zmeta, c
         cs[r][a] = sum;
     }
     __syncthreads();
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4;
+    const int wv = tid_x() >> 6, lane = tid_x() & 63, j = lane & 15, g = lane >> 4;
     const int x0 = blockIdx.x * 256 + wv * 64;
     if (x0 >= nx) return;
     int xc[4];
@@ -445,8 +456,8 @@ __device__ inline void row_store(float* orow, bool row_ok, int64_t n, int64_t nc
 template <bool VEC>
 __device__ inline void tile_out(float (*T)[TROW], const floatx4 (&acc)[4], float* __restrict__ oz, int64_t ncols,
                                 int ny, int yt, int64_t nwg0) {
-    const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
-    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int lane = tid_x() & 63, li = lane & 15, lk = lane >> 4;
+    const int wv = __builtin_amdgcn_readfirstlane(tid_x() >> 6);
 #pragma unroll
     for (int it = 0; it < 4; ++it) *reinterpret_cast<floatx4*>(&T[li][64 * wv + 16 * it + 4 * lk]) = acc[it];
     __syncthreads();
@@ -464,8 +475,8 @@ __device__ inline void tile_out(float (*T)[TROW], const floatx4 (&acc)[4], float
 // [yt0, yt1) are zero.
 template <bool VEC>
 __device__ inline void zero_tiles(float* __restrict__ oz, int64_t ncols, int ny, int yt0, int yt1, int64_t nwg0) {
-    const int lane = threadIdx.x & 63;
-    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int lane = tid_x() & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid_x() >> 6);
     const int64_t n = nwg0 + 4 * lane;
     const int yend = min(ny, 16 * yt1);
     for (int y = 16 * yt0 + wv; y < yend; y += 4)
@@ -490,7 +501,7 @@ __global__ __launch_bounds__(256) void main_f32(const ZMeta* __restrict__ zmeta,
     const int z = by + zoff;
     const ZMeta zm = zmeta[z];
     const int K16 = zm.kp >> 4;
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wv = tid_x() >> 6, lane = tid_x() & 63;
     const int64_t ncols = 6LL * nx;
     const int64_t n0 = (int64_t)bx * 256 + wv * 64;  // waves past ncols compute zeros
     const int li = lane & 15, lk = lane >> 4;
@@ -607,6 +618,22 @@ __device__ inline floatx4 mfma16(const typename Op16<MODE>::vec a, const typenam
 // contraction kernel did before it moved here.
 // ---------------------------------------------------------------------------
 // Element e = m ka + a of the slot-summed Cs_z of one z (m < 3 my, a < ka).
+// Element (m, a) of Cs_z for a region of few terms per cell (nslots = 0), not
+// contracted before: C_z[cell][comp] = sum_j A[j][cell][comp] Z_z[kz index of
+// term j], b = m / 3, comp = m % 3, cell = b mx + a (zero past mx).
+__device__ inline float2 folded_elem(const ZMeta& zm, const RegionDev& R, const float2* __restrict__ Z, int m, int a) {
+    float2 sum = make_float2(0.0f, 0.0f);
+    if (a >= zm.mx) return sum;
+    const int b = m / 3, comp = m - 3 * b, cell = b * zm.mx + a;
+    for (int j = 0; j < R.width; ++j) {
+        const float2 av = R.ellA[(int64_t)j * R.arow + (int64_t)cell * 3 + comp];
+        const float2 zv = Z[zm.zrow_off + R.ellC[(int64_t)j * R.cells + cell]];
+        sum.x += av.x * zv.x - av.y * zv.y;
+        sum.y += av.x * zv.y + av.y * zv.x;
+    }
+    return sum;
+}
+
 template <bool COH>
 __device__ inline float2 csum_elem(const ZMeta& zm, const RegionDev& R, const float2* __restrict__ C,
                                    int64_t c_elems, const float2* __restrict__ Z, int e) {
@@ -614,18 +641,7 @@ __device__ inline float2 csum_elem(const ZMeta& zm, const RegionDev& R, const fl
     const int m = e / ka, a = e - m * ka;
     float2 sum = make_float2(0.0f, 0.0f);
     if (a >= zm.mx) return sum;
-    if (ns == 0) {
-        // a region of few terms per cell, not contracted before: sum its terms
-        // here, C_z[cell][comp] = sum_j A[j][cell][comp] Z_z[kz index of term j]
-        const int b = m / 3, comp = m - 3 * b, cell = b * zm.mx + a;
-        for (int j = 0; j < R.width; ++j) {
-            const float2 av = R.ellA[(int64_t)j * R.arow + (int64_t)cell * 3 + comp];
-            const float2 zv = Z[zm.zrow_off + R.ellC[(int64_t)j * R.cells + cell]];
-            sum.x += av.x * zv.x - av.y * zv.y;
-            sum.y += av.x * zv.y + av.y * zv.x;
-        }
-        return sum;
-    }
+    if (ns == 0) return folded_elem(zm, R, Z, m, a);
     const int64_t off = zm.c_off + ((int64_t)(m / 3) * zm.mx + a) * 3 + (m % 3);
     for (int k0 = 0; k0 < ns; k0 += 16) {
         float2 v[16];
@@ -650,7 +666,7 @@ __global__ __launch_bounds__(256) void csum_kernel(const ZMeta* __restrict__ zme
     const int2 blk = blocks[blockIdx.x];  // (z, first element): only the elements each z has
     const ZMeta zm = zmeta[blk.x];
     const RegionDev& R = regions[zm.region];
-    const int e = blk.y + (int)threadIdx.x;
+    const int e = blk.y + tid_x();
     if (e >= 3 * zm.my * R.ka) return;
     Cs[zm.u_off + e] = csum_elem<false>(zm, R, C, c_elems, Z, e);
 }
@@ -710,11 +726,24 @@ __device__ inline void xc_mfma(floatx4& acc, const bf16x8& ah, const bf16x8& al,
 // first stages the X fragments, already split, in LDS (xs), and each row tile
 // issues all its C loads at once: one L2 round trip per row tile instead of
 // one per K step; larger mx takes the K steps one at a time.
+// Four consecutive elements a0 .. a0 + 3 of row m of Cs_z.  Ordinary loads
+// also in the single-launch kernel (COH): its main units read Cs_z only after
+// the slot sums of z are complete, and every Cs_z starts on its own 256-byte
+// boundary, so no cache holds a line of it from before (caches start clean
+// at launch); the lines then stay in the L2 for the other column blocks of z.
+__device__ inline void cs_row4(const ZMeta& zm, const RegionDev& R, const float2* __restrict__ Cz, int m, int a0,
+                               floatx4& c01, floatx4& c23) {
+    const float2* cp = Cz + (int64_t)m * R.ka + a0;
+    c01 = get4<false>(cp);
+    c23 = get4<false>(cp + 2);
+}
+
 template <int XS, bool COH>
 __device__ __attribute__((always_inline)) inline void xcontract_columns(const ZMeta& zm, const RegionDev& R, const float2* __restrict__ Cs,
-                                         const float2* __restrict__ X, int nx, int bx, float2 (*Ul)[UPL], XFrag* xs) {
-    const int lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4, o = j & 1;
-    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+                                         const float2* __restrict__ X, int nx, int bx, float2 (*Ul)[UPL],
+                                         XFrag* xs) {
+    const int lane = tid_x() & 63, j = lane & 15, g = lane >> 4, o = j & 1;
+    const int wv = __builtin_amdgcn_readfirstlane(tid_x() >> 6);
     const int p0 = 128 * bx, x_lo = p0 / 3, r0 = p0 - 3 * x_lo;
     const int M = 3 * zm.my, RT = (M + 7) >> 3, T = R.ka >> 4;
     const float2* Cz = Cs + zm.u_off;
@@ -744,7 +773,7 @@ __device__ __attribute__((always_inline)) inline void xcontract_columns(const ZM
         const int nf = 3 * T * 64;
 #pragma unroll
         for (int it = 0; it < (3 * (XS > 0 ? XS : 1) * 64 + 255) / 256; ++it) {
-            const int f = threadIdx.x + 256 * it, fc = min(f, nf - 1);
+            const int f = tid_x() + 256 * it, fc = min(f, nf - 1);
             const int l = fc & 63, ct = (fc >> 6) % 3, t = (fc >> 6) / 3;
             const int x = min(x_lo + 16 * ct + (l & 15), nx - 1);
             const floatx4* xp = reinterpret_cast<const floatx4*>(Xr + (int64_t)x * R.ka + 16 * t + 4 * (l >> 4));
@@ -760,11 +789,8 @@ __device__ __attribute__((always_inline)) inline void xcontract_columns(const ZM
             const int m = min(8 * rt + (j >> 1), M - 1);
             floatx4 cv[XS > 0 ? XS : 1][2];
 #pragma unroll
-            for (int t = 0; t < XS; ++t) {  // unconditional: steps past T re-read the last one
-                const float2* cp = Cz + (int64_t)m * R.ka + 16 * min(t, T - 1) + 4 * g;
-                cv[t][0] = get4<COH>(cp);
-                cv[t][1] = get4<COH>(cp + 2);
-            }
+            for (int t = 0; t < XS; ++t)  // unconditional: steps past T re-read the last one
+                cs_row4(zm, R, Cz, m, 16 * min(t, T - 1) + 4 * g, cv[t][0], cv[t][1]);
             floatx4 acc[3];
 #pragma unroll
             for (int ct = 0; ct < 3; ++ct) acc[ct] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
@@ -788,12 +814,13 @@ __device__ __attribute__((always_inline)) inline void xcontract_columns(const ZM
 #pragma unroll
             for (int ct = 0; ct < 3; ++ct) acc[ct] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
             for (int t = 0; t < T; ++t) {
-                const float2* cp = Cz + (int64_t)m * R.ka + 16 * t + 4 * g;
                 floatx4 xv[3][2];
 #pragma unroll
                 for (int ct = 0; ct < 3; ++ct) xload(ct, t, xv[ct][0], xv[ct][1]);
+                floatx4 c01, c23;
+                cs_row4(zm, R, Cz, m, 16 * t + 4 * g, c01, c23);
                 bf16x8 ah, al;
-                xc_a_operand(get4<COH>(cp), get4<COH>(cp + 2), o, ah, al);
+                xc_a_operand(c01, c23, o, ah, al);
 #pragma unroll
                 for (int ct = 0; ct < 3; ++ct) {
                     bf16x8 bh, bl;
@@ -827,7 +854,7 @@ __device__ inline void main_16_body(int bx, int z, int bz, const ZMeta* __restri
     constexpr int S = KMAX / 32;
     const ZMeta zm = zmeta[z];
     const int K32 = zm.kp >> 5;
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wv = tid_x() >> 6, lane = tid_x() & 63;
     const int64_t ncols = 6LL * nx;
     const int64_t n0 = (int64_t)bx * 256 + wv * 64;  // waves past ncols compute zeros
     const int li = lane & 15, lk = lane >> 4;
@@ -980,7 +1007,7 @@ __global__ __launch_bounds__(256) void bfp_encode(const float2* __restrict__ A, 
                                                   int mcols, int64_t groups, uint32x4* __restrict__ mant,
                                                   uint32_t* __restrict__ expo) {
     const int64_t n = groups * erow;
-    for (int64_t i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    for (int64_t i = blockIdx.x * 256LL + tid_x(); i < n; i += (int64_t)gridDim.x * 256) {
         const int64_t q = i / erow, cell = i - q * erow;
         float2 a[4][3];
         int s[4];
@@ -1103,7 +1130,7 @@ int build_bfp(scw_plan* P) {
 // magnitude from its phase factor into its amplitudes this way.
 __global__ __launch_bounds__(256) void bfp_shift_exp(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
                                                      int64_t erow, int64_t n, const int32_t* __restrict__ shift) {
-    for (int64_t i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    for (int64_t i = blockIdx.x * 256LL + tid_x(); i < n; i += (int64_t)gridDim.x * 256) {
         const int64_t q = i / erow;
         const uint32_t w = src[i];
         uint32_t o = 0;
@@ -1148,7 +1175,7 @@ int shift_bfp_exponents(const uint8_t* src, uint8_t* dst, int64_t erow, int64_t 
 template <int MT, bool COH>
 __device__ inline void modal_store(const floatx4 (&acc)[4][MT], const ModalItem& I, const ZMeta* __restrict__ zmeta,
                                    int64_t c_elems, float2* __restrict__ C, int mcols) {
-    const int lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4;
+    const int lane = tid_x() & 63, j = lane & 15, g = lane >> 4;
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
@@ -1167,7 +1194,7 @@ __device__ inline void modal_store(const floatx4 (&acc)[4][MT], const ModalItem&
 template <int MODE, int MT, bool COH>
 __device__ inline void modal_body(const ModalItem& I, const RegionDev& R, const ZMeta* __restrict__ zmeta,
                                   const uint32_t* __restrict__ zf, int64_t c_elems, float2* __restrict__ C) {
-    const int lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4;
+    const int lane = tid_x() & 63, j = lane & 15, g = lane >> 4;
     const int mcols = 3 * R.cells;
     const int W = R.width;
     const float2* A = R.ellA;
@@ -1294,8 +1321,8 @@ __device__ inline void modal_body_wg(const ModalItem& I, const ModalItem& L, con
     constexpr int MT = 8, NH = MODE == 1 ? 2 : 1;  // fragment halves: high (and low)
     typedef typename Op16<MODE>::vec vec;
     uint32x4 (*zs)[MT][NH][64] = reinterpret_cast<uint32x4 (*)[MT][NH][64]>(lds);
-    const int lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4;
-    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int lane = tid_x() & 63, j = lane & 15, g = lane >> 4;
+    const int wv = __builtin_amdgcn_readfirstlane(tid_x() >> 6);
     const int mcols = 3 * R.cells;
     int col[4], ecol[4];  // column and its cell (block floating-point exponent word)
 #pragma unroll
@@ -1392,7 +1419,7 @@ __global__ __launch_bounds__(256) void modal_kernel(const ModalItem* __restrict_
     // XCD-contiguous order for the short-run class only: the FP32 64-z class
     // measured slower with it (64-slab batch, 3.65 -> 4.29 ms)
     const int lb = MT == 1 ? xcd_block() : (int)blockIdx.x;
-    const int it = __builtin_amdgcn_readfirstlane(lb * 4 + (int)(threadIdx.x >> 6));
+    const int it = __builtin_amdgcn_readfirstlane(lb * 4 + (int)(tid_x() >> 6));
     if (it >= nitems) return;
     const ModalItem I = items[it];
     modal_body<MODE, MT, false>(I, regions[I.region], zmeta, zf, c_elems, C);
@@ -1409,7 +1436,7 @@ __global__ __launch_bounds__(256, 2) void modal_wg_kernel(const ModalItem* __res
     // range comes from the workgroup's first item, so every wave passes the
     // same barriers.
     if ((int)blockIdx.x * 4 + 3 >= nitems) return;
-    const int it = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+    const int it = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (tid_x() >> 6)));
     const ModalItem I = items[it];
     __shared__ __attribute__((aligned(16))) unsigned char lds[ModalWgLds<MODE>::kBytes];
     modal_body_wg<MODE, false>(I, items[blockIdx.x * 4], regions[I.region], zmeta, zf, c_elems, C, lds);
@@ -1440,9 +1467,262 @@ static void launch_modal(const scw_exec* ex, hipStream_t st) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Single-launch form of the 16-bit dtypes: one persistent kernel runs every
+// stage.  Its work units (a table built on the host, in dependency order) are
+//   coefficient units: four wave items of the VALU contraction or of one
+//     MFMA mode-contraction class (a 64-z workgroup for the 64-z class);
+//   slot-sum units: one slot-sum workgroup (z, 256 elements of C_z);
+//   main units: one main-GEMM workgroup (column block, z, y split).
+// Workgroups take units in table order from a device counter (the first one
+// is blockIdx.x), so a unit only ever waits for units taken before it, by
+// workgroups that are running: the slot sum of z waits until every wave item
+// covering z has raised done[z] to need[z]; a main unit of z until the slot
+// sums of z have raised csd[z] to their count.  Data handed between units
+// goes through sc1 accesses (put2 / get2 with COH), and a producer raises its
+// counter only after its stores have completed.  Every wait is bounded by a
+// wall-clock limit; one that expires sets the error word (the unit then runs
+// on whatever it reads, so the launch still ends) for the host to check.  The
+// last workgroup to finish resets the counters, so the next launch (or graph
+// replay) starts from zero without a memset.
+// ---------------------------------------------------------------------------
+enum : int { kUnitCoef = 0, kUnitModal1 = 1, kUnitModal2 = 2, kUnitModal4 = 3, kUnitModalWg = 4, kUnitCsum = 5, kUnitMain = 6 };
+constexpr int kCtlHead = 0, kCtlExit = 1, kCtlErr = 2, kCtlWords = 4;  // then done[nz], csd[nz]
+
+struct FusedArgs {
+    const int4* units;  // (kind, first item | cblk index | column block, item end | z, y split)
+    int nunits;
+    int nz;
+    const CoefItem* items;
+    const ModalItem* mitems;
+    const RegionDev* regions;
+    const ZMeta* zmeta;
+    const float2* Z;
+    const uint32_t* zf;
+    int64_t c_elems;
+    float2* C;
+    float2* Cs;
+    const int2* cblk;
+    const float2* X;
+    const uint16_t* Yfrag;
+    int64_t ylo_delta;
+    int nx, ny, ytiles_per_split;
+    float* out;
+    int* ctl;          // kCtlWords control words, then done[nz], csd[nz]
+    const int* need;   // need[nz] wave items per z, then the slot-sum units per z
+    uint64_t wait_ticks;  // wall-clock ticks a wait may last
+    uint64_t wait_polls;  // and its most polls
+};
+
+__device__ inline int load_count(const int* p) {
+    return __hip_atomic_load(const_cast<int*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void raise_count(int* p) { __hip_atomic_fetch_add(p, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+// Waits (one thread) until *p >= target, at most `ticks` of the wall clock
+// and at most `polls` polls (a bound that holds whatever the clock does).
+__device__ inline bool wait_count(const int* p, int target, uint64_t ticks, uint64_t polls) {
+    if (load_count(p) >= target) return true;
+    const uint64_t t0 = wall_clock64();
+    for (uint64_t n = 0; n < polls; ++n) {
+        __builtin_amdgcn_s_sleep(2);
+        if (load_count(p) >= target) return true;
+        if (wall_clock64() - t0 > ticks) return false;
+    }
+    return false;
+}
+
+// A wave item's z run is complete: its stores first, then done[z] per z.
+__device__ inline void items_done(int* done, int z0, int nz) {
+    stores_done();
+    const int lane = tid_x() & 63;
+    if (lane < nz) raise_count(done + z0 + lane);
+}
+
+template <int KMAX, int MODE>
+struct FusedLds {
+    static constexpr int kA = Main16Lds<KMAX>::kBytes, kB = ModalWgLds<MODE>::kBytes;
+    static constexpr int kBytes = kA > kB ? kA : kB;
+};
+
+template <int KMAX, bool VEC, int MODE>
+__global__ __launch_bounds__(256, 2) void fused16(const FusedArgs a) {
+    __shared__ __attribute__((aligned(16))) unsigned char lds[FusedLds<KMAX, MODE>::kBytes];
+    __shared__ int s_unit[2], s_last;
+    int* const done = a.ctl + kCtlWords;
+    int* const csd = done + a.nz;
+    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    int next = (int)blockIdx.x;  // thread 0: the unit taken for the next iteration
+    for (int it = 0;; ++it) {
+        // Thread 0 waits for what its unit needs; the barrier hands the unit
+        // to the other waves (double-buffered, so one barrier per unit
+        // suffices).
+        if (threadIdx.x == 0) {
+            const int u = next;
+            if (u < a.nunits) {
+                const int4 U = a.units[u];
+                const int* p = nullptr;
+                int target = 0;
+                if (U.x == kUnitCsum) {
+                    const int z = a.cblk[U.y].x;
+                    p = done + z;
+                    target = a.need[z];
+                } else if ((U.x & 0xff) == kUnitMain) {
+                    p = csd + U.z;
+                    target = a.need[a.nz + U.z];
+                }
+                if (p && !wait_count(p, target, a.wait_ticks, a.wait_polls))
+                    __hip_atomic_fetch_or(a.ctl + kCtlErr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            s_unit[it & 1] = u;
+        }
+        __syncthreads();
+        const int u = __builtin_amdgcn_readfirstlane(s_unit[it & 1]);
+        if (u >= a.nunits) break;
+        // take the next unit now: the counter's round trip overlaps this one
+        if (threadIdx.x == 0) next = (int)gridDim.x + atomicAdd(a.ctl + kCtlHead, 1);
+        const int4 U = a.units[u];
+        switch (U.x & 0xff) {
+            case kUnitCoef: {
+                const int i = U.y + wv;
+                if (i < U.z) {
+                    const CoefItem I = a.items[i];
+                    coef_body<true>(I, a.regions, a.zmeta, a.Z, a.c_elems, a.C);
+                    items_done(done, I.z0, I.nz);
+                }
+                break;
+            }
+            case kUnitModal1:
+            case kUnitModal2:
+            case kUnitModal4: {
+                const int i = U.y + wv;
+                if (i < U.z) {
+                    const ModalItem I = a.mitems[i];
+                    const RegionDev& R = a.regions[I.region];
+                    if (U.x == kUnitModal1) modal_body<MODE, 1, true>(I, R, a.zmeta, a.zf, a.c_elems, a.C);
+                    else if (U.x == kUnitModal2) modal_body<MODE, 2, true>(I, R, a.zmeta, a.zf, a.c_elems, a.C);
+                    else modal_body<MODE, 4, true>(I, R, a.zmeta, a.zf, a.c_elems, a.C);
+                    items_done(done, I.z0, I.nz);
+                }
+                break;
+            }
+            case kUnitModalWg: {
+                const ModalItem I = a.mitems[U.y + wv];
+                modal_body_wg<MODE, true>(I, a.mitems[U.y], a.regions[I.region], a.zmeta, a.zf, a.c_elems, a.C, lds);
+                items_done(done, I.z0, I.nz);
+                break;
+            }
+            case kUnitCsum: {  // U.z consecutive slot-sum blocks of one z
+                const int z = a.cblk[U.y].x;
+                const ZMeta zm = a.zmeta[z];
+                const RegionDev& R = a.regions[zm.region];
+                for (int k = 0; k < U.z; ++k) {
+                    const int e = a.cblk[U.y + k].y + tid_x();
+                    if (e < 3 * zm.my * R.ka)
+                        put2<true>(a.Cs + zm.u_off + e, csum_elem<true>(zm, R, a.C, a.c_elems, a.Z, e));
+                }
+                stores_done();
+                __syncthreads();
+                if (threadIdx.x == 0) raise_count(csd + z);
+                break;
+            }
+            default:  // U.x >> 8 consecutive column blocks from U.y of z-slice U.z, y split U.w
+                for (int t = 0; t < (U.x >> 8); ++t) {
+                    if (t > 0) __syncthreads();  // the previous block's last transpose is read
+                    main_16_body<KMAX, VEC, MODE, true>(U.y + t, U.z, U.w, a.zmeta, a.regions, a.Cs, a.X,
+                                                        a.Yfrag, a.ylo_delta, a.nx, a.ny, a.ytiles_per_split, a.out,
+                                                        lds);
+                }
+                break;
+        }
+    }
+    // Every workgroup ends here once the units are taken; the last one
+    // resets the counters for the next launch.  Each wave first waits for its
+    // counter increments (issued without a return value, still in flight
+    // otherwise), so no increment can land after the reset.
+    stores_done();
+    __syncthreads();
+    if (threadIdx.x == 0) s_last = atomicAdd(a.ctl + kCtlExit, 1) == (int)gridDim.x - 1;
+    __syncthreads();
+    if (s_last) {
+        for (int i = (int)threadIdx.x; i < 2 * a.nz; i += 256)
+            __hip_atomic_store(done + i, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (threadIdx.x == 0) {
+            __hip_atomic_store(a.ctl + kCtlHead, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(a.ctl + kCtlExit, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+template <int KMAX, bool VEC, int MODE>
+static int fused_kernel_occupancy(int* per_cu) {
+    SCW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fused16<KMAX, VEC, MODE>, 256, 0));
+    return SCW_OK;
+}
+
+static FusedArgs fused_args(const scw_exec* ex, float* out) {
+    FusedArgs a;
+    a.units = ex->d_units.as<int4>();
+    a.nunits = (int)ex->nunits;
+    a.nz = (int)ex->nz;
+    a.items = ex->d_items.as<CoefItem>();
+    a.mitems = ex->d_mitems.as<ModalItem>();
+    a.regions = ex->d_regions.as<RegionDev>();
+    a.zmeta = ex->d_zmeta.as<ZMeta>();
+    a.Z = ex->d_Z.as<float2>();
+    a.zf = ex->d_Zf.as<uint32_t>();
+    a.c_elems = ex->c_elems;
+    a.C = ex->d_C.as<float2>();
+    a.Cs = ex->d_U.as<float2>();
+    a.cblk = ex->d_cblk.as<int2>();
+    a.X = ex->d_X.as<float2>();
+    a.Yfrag = ex->d_Yfrag.as<uint16_t>();
+    a.ylo_delta = ex->ylo_delta;
+    a.nx = (int)ex->nx;
+    a.ny = (int)ex->ny;
+    a.ytiles_per_split = ex->ytiles_per_split;
+    a.out = out;
+    a.ctl = ex->d_ctl.as<int>();
+    a.need = ex->d_need.as<int>();
+    a.wait_ticks = ex->wait_ticks;
+    a.wait_polls = ex->wait_polls;
+    return a;
+}
+
+template <int MODE>
+static int dispatch_fused(const scw_exec* ex, float* out, hipStream_t st, int* per_cu) {
+    const bool vec = (6 * ex->nx) % 4 == 0;
+#define SCW_FUSED(K)                                                                                        \
+    if (per_cu) return vec ? fused_kernel_occupancy<K, true, MODE>(per_cu) : fused_kernel_occupancy<K, false, MODE>(per_cu); \
+    if (vec) hipLaunchKernelGGL((fused16<K, true, MODE>), dim3((unsigned)ex->pgrid), dim3(256), 0, st, fused_args(ex, out)); \
+    else hipLaunchKernelGGL((fused16<K, false, MODE>), dim3((unsigned)ex->pgrid), dim3(256), 0, st, fused_args(ex, out)); \
+    break;
+    switch (ex->kmax) {
+        case 32: SCW_FUSED(32)
+        case 64: SCW_FUSED(64)
+        case 96: SCW_FUSED(96)
+        default: SCW_FUSED(128)
+    }
+#undef SCW_FUSED
+    SCW_HIP(hipGetLastError());
+    return SCW_OK;
+}
+
+// Resident workgroups per CU of the single-launch kernel for this exec, or
+// the launch itself (per_cu == nullptr).
+int fused_launch(const scw_exec* ex, float* out, hipStream_t st, int* per_cu) {
+    switch (ex->dtype) {
+        case SCW_DTYPE_BF16: return dispatch_fused<0>(ex, out, st, per_cu);
+        case SCW_DTYPE_BF16X3: return dispatch_fused<1>(ex, out, st, per_cu);
+        case SCW_DTYPE_FP16: return dispatch_fused<2>(ex, out, st, per_cu);
+        default: return fail(SCW_ERR_UNSUPPORTED, "single-launch form: 16-bit dtypes only");
+    }
+}
+
 // Stages: 1 = coefficient contraction (VALU ELL items and MFMA mode items),
 // 2 = X contraction, 3 = main MFMA GEMM; 0 = all three.
 int launch_gemm(const scw_exec* ex, float* out, hipStream_t st, int stage) {
+    if (stage == 0 && ex->persistent) return fused_launch(ex, out, st, nullptr);
     const int nitems = (int)ex->items.size();
     const int nmitems = (int)ex->mitems.size();
     if (stage == 0 || stage == 1) {

@@ -209,6 +209,19 @@ struct scw_exec {
     int64_t ylo_delta = 0;    // element offset of the lo Y fragments (bf16 x3)
     int ysplit = 1;           // grid.z of the main GEMM
     int ytiles_per_split = 1;
+    // single-launch form (16-bit dtypes, scw_exec_set_persistent)
+    bool persistent = false;
+    int64_t nunits = 0;       // work units of the persistent kernel
+    int64_t pgrid = 0;        // its workgroups
+    int64_t tune_grid = 0;    // requested workgroups (0 = resident capacity)
+    int64_t order_lag = -1;   // unit-order lag (-1 = default)
+    int64_t main_tiles = 0;   // main-GEMM column blocks per unit (0 = default)
+    double wait_ms = 0;       // limit of one dependency wait
+    uint64_t wait_ticks = 0;  // the same in wall-clock ticks
+    uint64_t wait_polls = 0;  // poll bound of a wait (10^5 per ms, each poll sleeps >= 128 cycles)
+    bool stall = false;       // test hook: one wait can never be met
+    std::vector<int2> hcblk;  // slot-sum workgroups (z, first element), host copy
+    scw::DevBuf d_units, d_need, d_ctl;
     scw::DevBuf d_out;        // staging for scw_exec_run_host
     // graph
     hipGraph_t graph = nullptr;
@@ -231,6 +244,9 @@ int launch_direct_raw(const scw_plan* P, const void* region_table, const int32_t
                       const float* dxs, const float* dys, int64_t nx, int64_t ny, int64_t nz, double xc, double yc,
                       int ppt, int nsplit, float* part, float* out, hipStream_t st);
 int launch_gemm(const scw_exec* ex, float* out, hipStream_t s, int stage = 0);
+// the single-launch kernel: launch (per_cu == nullptr) or its resident
+// workgroups per CU (scw_gemm.hip)
+int fused_launch(const scw_exec* ex, float* out, hipStream_t s, int* per_cu);
 int build_bfp(scw_plan* plan);  // fills plan->d_bfp / bfp_off once (scw_gemm.hip)
 // (re)builds a single-plan exec in place (scw_exec.cpp); on failure it is left invalid
 int exec_init(scw_exec* ex, scw_plan* plan, const double* xs, size_t nx, const double* ys, size_t ny,

@@ -121,6 +121,22 @@ class Exec:
         """GEMM-form launch shape (0 = default); rebuilds the work lists."""
         check(_native.lib().scw_exec_set_launch(self.handle, int(coef_items), int(main_blocks)))
 
+    def set_persistent(self, on=True, workgroups=0, wait_ms=0.0):
+        """Single-launch form (GEMM form, 16-bit dtypes): every run is one
+        persistent kernel; workgroups 0 = resident capacity, wait_ms 0 = 10 s."""
+        check(_native.lib().scw_exec_set_persistent(self.handle, int(bool(on)), int(workgroups), float(wait_ms)))
+
+    def persistent_info(self):
+        on, units, wg = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int64()
+        check(_native.lib().scw_exec_persistent_info(self.handle, ctypes.byref(on), ctypes.byref(units),
+                                                     ctypes.byref(wg)))
+        return dict(on=bool(on.value), units=units.value, workgroups=wg.value)
+
+    def check(self, stream=None):
+        """Wait for `stream`; raise ScwError(SCW_ERR_TIMEOUT) if a wait of a
+        single-launch run expired since the last check."""
+        check(_native.lib().scw_exec_check(self.handle, stream))
+
     def graph_capture(self, out_ptr, stream=None):
         check(_native.lib().scw_exec_graph_capture(self.handle, out_ptr, stream))
 
