@@ -103,7 +103,7 @@ static int stage_launches(const scw_exec* ex) {
     int n = ex->items.empty() ? 0 : 1;
     for (int k = 0; k < 4; ++k) n += ex->mclass_off[k + 1] > ex->mclass_off[k] ? 1 : 0;
     const bool xstage = ex->dtype == SCW_DTYPE_FP32 ? ex->my_max > 0 : ex->ncblk > 0;  // u_mfma / slot sum
-    return n + (xstage ? 1 : 0) + 1;
+    return n + (xstage ? 1 : 0) + (int)std::max<size_t>(1, ex->mlevel_off.size() - 1);
 }
 
 // Work units of the single-launch kernel (kinds and fields as fused16 in
@@ -249,6 +249,11 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
     std::memset(rd.data(), 0, rd.size() * sizeof(RegionDev));
     std::vector<float> X;  // float2 pairs
     std::vector<int64_t> yfrag_off(NV, 0);
+    struct Chunk {
+        int b0, my, kp;
+        int64_t yfrag_off;
+    };
+    std::vector<std::vector<Chunk>> chunks(NV);
     std::vector<float> yf32;
     std::vector<uint16_t> yhi, ylo;
     int kmax = 0;
@@ -295,48 +300,60 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
                 X[2 * e] = (float)val.real();
                 X[2 * e + 1] = (float)val.imag();
             }
-        // Y fragments: Yp[y][2b] = Re exp(i ky_b y), Yp[y][2b+1] = Im.
-        const int kp = R.my > 0 ? round_up(2 * R.my, kq) : 0;
-        kmax = std::max(kmax, kp);
-        std::vector<double> Yp((size_t)ny * std::max(kp, 1), 0.0);
-        for (int j = 0; j < ny; ++j)
-            for (int b = 0; b < R.my; ++b) {
-                const cd val = std::exp(cd(0, 1) * R.ky[b] * ys[j]);
-                if (!region_ok(val)) return fail(SCW_ERR_UNSUPPORTED, "region %d: exp(i ky y) outside FP32 range", r);
-                Yp[(size_t)j * kp + 2 * b] = val.real();
-                Yp[(size_t)j * kp + 2 * b + 1] = val.imag();
-            }
-        auto yval = [&](int y, int k) { return (y < ny && k < kp) ? Yp[(size_t)y * kp + k] : 0.0; };
-        if (!is16) {
-            // [yt][q][lane][e]: lane l holds k = 4 (4q + e) + (l >> 4) of row 16 yt + (l & 15)
-            yfrag_off[v] = (int64_t)yf32.size();
-            const int K16 = kp / 16;
-            for (int yt = 0; yt < ytiles; ++yt)
-                for (int q = 0; q < K16; ++q)
-                    for (int l = 0; l < 64; ++l)
-                        for (int e = 0; e < 4; ++e)
-                            yf32.push_back((float)yval(16 * yt + (l & 15), 4 * (4 * q + e) + (l >> 4)));
-        } else {
-            // [yt][s][lane][e]: lane l holds k = 32 s + 8 (l >> 4) + e of row 16 yt + (l & 15)
-            yfrag_off[v] = (int64_t)yhi.size();
-            const int K32 = kp / 32;
-            for (int yt = 0; yt < ytiles; ++yt)
-                for (int s2 = 0; s2 < K32; ++s2)
-                    for (int l = 0; l < 64; ++l)
-                        for (int e = 0; e < 8; ++e) {
-                            const double val = yval(16 * yt + (l & 15), 32 * s2 + 8 * (l >> 4) + e);
-                            if (ex->dtype == SCW_DTYPE_FP16) {
-                                yhi.push_back(f16_rne((float)val));
-                                ylo.push_back(0);
-                            } else {
-                                const uint16_t h = bf16_rne((float)val);
-                                yhi.push_back(h);
-                                ylo.push_back(bf16_rne((float)(val - (double)bf16_f32(h))));
-                            }
-                        }
+        // ky chunks of the main GEMM: K = 2 my <= 128 per launch, so a region
+        // of more than 64 ky runs it over balanced chunks of ky rows
+        const int nch = R.my > 64 ? (R.my + 63) / 64 : 1;
+        const int myc = (R.my + nch - 1) / nch;
+        for (int c = 0; c < nch; ++c) {
+            const int b0 = c * myc, bc = std::max(0, std::min(myc, R.my - b0));
+            chunks[v].push_back({b0, bc, bc > 0 ? round_up(2 * bc, kq) : 0, 0});
         }
+        // Y fragments per chunk: Yp[y][2b] = Re exp(i ky_(b0 + b) y), Yp[y][2b+1] = Im.
+        for (Chunk& ch : chunks[v]) {
+            const int kp = ch.kp;
+            kmax = std::max(kmax, kp);
+            std::vector<double> Yp((size_t)ny * std::max(kp, 1), 0.0);
+            for (int j = 0; j < ny; ++j)
+                for (int b = 0; b < ch.my; ++b) {
+                    const cd val = std::exp(cd(0, 1) * R.ky[ch.b0 + b] * ys[j]);
+                    if (!region_ok(val))
+                        return fail(SCW_ERR_UNSUPPORTED, "region %d: exp(i ky y) outside FP32 range", r);
+                    Yp[(size_t)j * kp + 2 * b] = val.real();
+                    Yp[(size_t)j * kp + 2 * b + 1] = val.imag();
+                }
+            auto yval = [&](int y, int k) { return (y < ny && k < kp) ? Yp[(size_t)y * kp + k] : 0.0; };
+            if (!is16) {
+                // [yt][q][lane][e]: lane l holds k = 4 (4q + e) + (l >> 4) of row 16 yt + (l & 15)
+                ch.yfrag_off = (int64_t)yf32.size();
+                const int K16 = kp / 16;
+                for (int yt = 0; yt < ytiles; ++yt)
+                    for (int q = 0; q < K16; ++q)
+                        for (int l = 0; l < 64; ++l)
+                            for (int e = 0; e < 4; ++e)
+                                yf32.push_back((float)yval(16 * yt + (l & 15), 4 * (4 * q + e) + (l >> 4)));
+            } else {
+                // [yt][s][lane][e]: lane l holds k = 32 s + 8 (l >> 4) + e of row 16 yt + (l & 15)
+                ch.yfrag_off = (int64_t)yhi.size();
+                const int K32 = kp / 32;
+                for (int yt = 0; yt < ytiles; ++yt)
+                    for (int s2 = 0; s2 < K32; ++s2)
+                        for (int l = 0; l < 64; ++l)
+                            for (int e = 0; e < 8; ++e) {
+                                const double val = yval(16 * yt + (l & 15), 32 * s2 + 8 * (l >> 4) + e);
+                                if (ex->dtype == SCW_DTYPE_FP16) {
+                                    yhi.push_back(f16_rne((float)val));
+                                    ylo.push_back(0);
+                                } else {
+                                    const uint16_t h = bf16_rne((float)val);
+                                    yhi.push_back(h);
+                                    ylo.push_back(bf16_rne((float)(val - (double)bf16_f32(h))));
+                                }
+                            }
+            }
+        }
+        yfrag_off[v] = chunks[v][0].yfrag_off;
     }
-    if (kmax > 128) return fail(SCW_ERR_UNSUPPORTED, "GEMM form supports at most 64 distinct ky per region");
+    if (kmax > 128) return fail(SCW_ERR_ARG, "internal: a ky chunk of the main GEMM past K = 128");
     if (!is16)
         ex->kmax = kmax <= 16 ? 16 : kmax <= 32 ? 32 : kmax <= 64 ? 64 : 128;
     else
@@ -474,8 +491,10 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
         m.region = v;
         m.my = R.my;
         m.mx = R.mx;
-        m.kp = R.my > 0 ? round_up(2 * R.my, kq) : 0;
+        m.kp = chunks[v][0].kp;
         m.u_off = u_off;
+        m.zout = z;
+        m.acc = 0;
         m.yfrag_off = yfrag_off[v];
         m.zrow_off = (int64_t)Z.size() / 2;
         m.c_off = c_off;
@@ -514,6 +533,28 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
         }
     }
     ex->c_elems = c_off;
+    // Main-GEMM entries: level L holds chunk L of every z whose region has
+    // one (level 0: every z, in z order); each level is one launch, and the
+    // levels after the first add to the output.
+    std::vector<ZMeta> mzm;
+    ex->mlevel_off.assign(1, 0);
+    size_t levels = 1;
+    for (int v = 0; v < NV; ++v) levels = std::max(levels, chunks[v].size());
+    for (size_t L = 0; L < levels; ++L) {
+        for (int z = 0; z < nz; ++z) {
+            const int v = zv[z];
+            if (chunks[v].size() <= L) continue;
+            const Chunk& ch = chunks[v][L];
+            ZMeta e = zm[z];
+            e.my = ch.my;
+            e.kp = ch.kp;
+            e.yfrag_off = ch.yfrag_off;
+            e.u_off = zm[z].u_off + (is16 ? 3LL * ch.b0 * rd[v].ka : (int64_t)ch.b0 * nx * 3);
+            e.acc = L > 0 ? 1 : 0;
+            mzm.push_back(e);
+        }
+        ex->mlevel_off.push_back((int64_t)mzm.size());
+    }
     // Slot-sum workgroups: 256 elements of one z's C_z each, per z only as
     // many as its region needs (a batch mixes regions of very different size).
     std::vector<int2> cblk;
@@ -705,6 +746,7 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
     int st;
     if ((st = ex->d_regions.upload(rd.data(), rd.size() * sizeof(RegionDev))) != SCW_OK) return st;
     if ((st = ex->d_zmeta.upload(zm.data(), zm.size() * sizeof(ZMeta))) != SCW_OK) return st;
+    if ((st = ex->d_mzmeta.upload(mzm.data(), mzm.size() * sizeof(ZMeta))) != SCW_OK) return st;
     if ((st = ex->d_items.upload(ex->items.data(), ex->items.size() * sizeof(CoefItem))) != SCW_OK) return st;
     if ((st = ex->d_mitems.upload(ex->mitems.data(), ex->mitems.size() * sizeof(ModalItem))) != SCW_OK) return st;
     if ((st = ex->d_Z.upload(Z.data(), Z.size() * sizeof(float))) != SCW_OK) return st;
@@ -742,8 +784,10 @@ static std::vector<ZEntry> entries(const scw_exec* ex) {
 // Drops the GEMM-form tables and work lists (before a fall back to the direct form).
 static void release_gemm(scw_exec* ex) {
     for (DevBuf* b : {&ex->d_items, &ex->d_mitems, &ex->d_regions, &ex->d_zmeta, &ex->d_Z, &ex->d_Zf, &ex->d_X, &ex->d_C,
-                      &ex->d_U, &ex->d_Yfrag, &ex->d_bfpx, &ex->d_cblk, &ex->d_units, &ex->d_need, &ex->d_ctl})
+                      &ex->d_U, &ex->d_Yfrag, &ex->d_bfpx, &ex->d_cblk, &ex->d_units, &ex->d_need, &ex->d_ctl,
+                      &ex->d_mzmeta})
         b->release();
+    ex->mlevel_off.clear();
     ex->persistent = false;
     ex->nunits = ex->pgrid = 0;
     ex->hcblk.clear();
@@ -762,7 +806,7 @@ static void exec_free(scw_exec* ex) {
     if (ex->graph_stream) (void)hipStreamDestroy(ex->graph_stream);
     for (DevBuf* b : {&ex->d_dx, &ex->d_dy, &ex->d_z, &ex->d_zslab, &ex->d_items, &ex->d_mitems, &ex->d_regions, &ex->d_zmeta,
                       &ex->d_Z, &ex->d_Zf, &ex->d_X, &ex->d_C, &ex->d_U, &ex->d_Yfrag, &ex->d_bfpx, &ex->d_cblk,
-                      &ex->d_units, &ex->d_need, &ex->d_ctl, &ex->d_out})
+                      &ex->d_units, &ex->d_need, &ex->d_ctl, &ex->d_mzmeta, &ex->d_out})
         b->release();
     delete ex;
 }
@@ -1005,6 +1049,8 @@ int scw_exec_set_persistent(scw_exec* ex, int on, int64_t workgroups, double wai
         return fail(SCW_ERR_ARG, "wait limit must be in [0, 3.6e6] ms (0 = default)");
     if (on && (ex->form != SCW_FORM_GEMM || ex->dtype == SCW_DTYPE_FP32))
         return fail(SCW_ERR_UNSUPPORTED, "single-launch form: GEMM form with a 16-bit dtype only");
+    if (on && ex->mlevel_off.size() > 2)
+        return fail(SCW_ERR_UNSUPPORTED, "single-launch form: a region has more than 64 ky (chunked main GEMM)");
     if (ex->invalid) return refuse_invalid(ex);
     DeviceGuard g(ex->plan->device);
     if (g.st != SCW_OK) return g.st;

@@ -293,7 +293,6 @@ __global__ __launch_bounds__(256) void coef_kernel(const CoefItem* __restrict__ 
     coef_body<false>(items[it], regions, zmeta, Z, c_elems, C);
 }
 
-constexpr int MX_MAX = 128;
 
 // ---------------------------------------------------------------------------
 // X contraction of the FP32 dtype on MFMA (v_mfma_f32_16x16x4_f32): per z,
@@ -306,13 +305,15 @@ constexpr int MX_MAX = 128;
 // dtypes run their X contraction inside the main GEMM.)
 // Workgroup = 8 b (24 rows m) x 256 x of one z; wave = 3 row tiles x 64 x.
 // ---------------------------------------------------------------------------
-constexpr int UB = 8;              // b per workgroup
-constexpr int UPITCH = MX_MAX + 2; // LDS row pitch in float2 (rows stay 16-byte aligned)
+constexpr int UB = 8;  // b per workgroup
 
+// PITCH: LDS row pitch in float2, ka + 2 for the largest ka (mx rounded up to
+// 16) it takes (rows stay 16-byte aligned): 130 up to 128 kx, 258 up to 256.
+template <int PITCH>
 __global__ __launch_bounds__(256) void u_mfma(const ZMeta* __restrict__ zmeta, const RegionDev* __restrict__ regions,
                                               const float2* __restrict__ C, int64_t c_elems,
                                               const float2* __restrict__ X, int nx, float2* __restrict__ U, int zoff) {
-    __shared__ float2 cs[3 * UB][UPITCH];
+    __shared__ float2 cs[3 * UB][PITCH];
     const int z = blockIdx.z + zoff;
     const ZMeta zm = zmeta[z];
     const int b0 = blockIdx.y * UB;
@@ -433,21 +434,26 @@ constexpr int TROW = 256 + 4;
 // the hardware range check drops the lanes and rows outside the field, so
 // the stores carry no branches and the wait counters see the same number of
 // stores on every path.
+// With acc (a ky chunk after the first), the values are added to what the
+// row holds (loads through the same range-checked resource).
 template <bool VEC>
-__device__ inline void row_store(float* orow, bool row_ok, int64_t n, int64_t ncols, const floatx4& v) {
+__device__ inline void row_store(float* orow, bool row_ok, int64_t n, int64_t ncols, const floatx4& v, bool acc) {
     const uint32_t nrec = row_ok ? (uint32_t)(ncols * 4) : 0u;
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(orow, 0, (int)nrec, kBufferRsrcWord3);
     const int off = (int)(n * 4);
     if (VEC) {
-        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(uint32x4, v), rs, off, 0, kStoreNT);
+        floatx4 w = v;
+        if (acc) w += __builtin_bit_cast(floatx4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(uint32x4, w), rs, off, 0, kStoreNT);
     } else {
         // Each element gets its own offset, moved far out of range past the
         // row's end, so a partly valid group of 4 columns is written element
         // by element and the stores are never merged into one 16-byte access.
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const float f = v[r];
+            float f = v[r];
             const int off_r = n + r < ncols ? off + 4 * r : 0x7ffffff0;
+            if (acc) f += __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off_r, 0, 0));
             __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, f), rs, off_r, 0, kStoreNT);
         }
     }
@@ -455,7 +461,7 @@ __device__ inline void row_store(float* orow, bool row_ok, int64_t n, int64_t nc
 
 template <bool VEC>
 __device__ inline void tile_out(float (*T)[TROW], const floatx4 (&acc)[4], float* __restrict__ oz, int64_t ncols,
-                                int ny, int yt, int64_t nwg0) {
+                                int ny, int yt, int64_t nwg0, bool add) {
     const int lane = tid_x() & 63, li = lane & 15, lk = lane >> 4;
     const int wv = __builtin_amdgcn_readfirstlane(tid_x() >> 6);
 #pragma unroll
@@ -468,7 +474,7 @@ __device__ inline void tile_out(float (*T)[TROW], const floatx4 (&acc)[4], float
     const int y0 = yt * 16 + wv;
 #pragma unroll
     for (int rr = 0; rr < 4; ++rr)
-        row_store<VEC>(oz + (int64_t)(y0 + 4 * rr) * ncols, y0 + 4 * rr < ny, n, ncols, v[rr]);
+        row_store<VEC>(oz + (int64_t)(y0 + 4 * rr) * ncols, y0 + 4 * rr < ny, n, ncols, v[rr], add);
 }
 
 // Slice of a region with no waves: the workgroup's columns of y tiles
@@ -480,7 +486,7 @@ __device__ inline void zero_tiles(float* __restrict__ oz, int64_t ncols, int ny,
     const int64_t n = nwg0 + 4 * lane;
     const int yend = min(ny, 16 * yt1);
     for (int y = 16 * yt0 + wv; y < yend; y += 4)
-        row_store<VEC>(oz + (int64_t)y * ncols, true, n, ncols, floatx4{0.0f, 0.0f, 0.0f, 0.0f});
+        row_store<VEC>(oz + (int64_t)y * ncols, true, n, ncols, floatx4{0.0f, 0.0f, 0.0f, 0.0f}, false);
 }
 
 // ---------------------------------------------------------------------------
@@ -498,8 +504,7 @@ __global__ __launch_bounds__(256) void main_f32(const ZMeta* __restrict__ zmeta,
     // (plain block order: the XCD-contiguous order of main_16 measured slower
     // here, 3.62 -> 3.94 ms on the 16-layer stack)
     const int bx = (int)blockIdx.x, by = (int)blockIdx.y, bz = (int)blockIdx.z;
-    const int z = by + zoff;
-    const ZMeta zm = zmeta[z];
+    const ZMeta zm = zmeta[by + zoff];  // main-GEMM entry
     const int K16 = zm.kp >> 4;
     const int wv = tid_x() >> 6, lane = tid_x() & 63;
     const int64_t ncols = 6LL * nx;
@@ -509,7 +514,7 @@ __global__ __launch_bounds__(256) void main_f32(const ZMeta* __restrict__ zmeta,
     const int ytiles = (ny + 15) >> 4;
     const int yt0 = bz * ytiles_per_split;
     const int yt1 = min(ytiles, yt0 + ytiles_per_split);
-    float* oz = out + (int64_t)z * ny * ncols;
+    float* oz = out + (int64_t)zm.zout * ny * ncols;
     if (K16 == 0) {
         zero_tiles<VEC>(oz, ncols, ny, yt0, yt1, (int64_t)bx * 256);
         return;
@@ -550,7 +555,7 @@ __global__ __launch_bounds__(256) void main_f32(const ZMeta* __restrict__ zmeta,
 #pragma unroll
                     for (int it = 0; it < 4; ++it)
                         acc[it] = __builtin_amdgcn_mfma_f32_16x16x4f32(uf[it][4 * q + e], y[q][e], acc[it], 0, 0, 0);
-        tile_out<VEC>(T[yt & 1], acc, oz, ncols, ny, yt, (int64_t)bx * 256);
+        tile_out<VEC>(T[yt & 1], acc, oz, ncols, ny, yt, (int64_t)bx * 256, zm.acc != 0);
     };
     // The first pair of tiles is peeled so that the loop is entered with the
     // same loads and stores in flight as at its back edge.
@@ -852,7 +857,7 @@ __device__ inline void main_16_body(int bx, int z, int bz, const ZMeta* __restri
     typedef typename Op16<MODE>::vec vec;
     typedef typename Op16<MODE>::elem elem;
     constexpr int S = KMAX / 32;
-    const ZMeta zm = zmeta[z];
+    const ZMeta zm = zmeta[z];  // main-GEMM entry
     const int K32 = zm.kp >> 5;
     const int wv = tid_x() >> 6, lane = tid_x() & 63;
     const int64_t ncols = 6LL * nx;
@@ -861,7 +866,7 @@ __device__ inline void main_16_body(int bx, int z, int bz, const ZMeta* __restri
     const int ytiles = (ny + 15) >> 4;
     const int yt0 = bz * ytiles_per_split;
     const int yt1 = min(ytiles, yt0 + ytiles_per_split);
-    float* oz = out + (int64_t)z * ny * ncols;
+    float* oz = out + (int64_t)zm.zout * ny * ncols;
     if (K32 == 0) {
         zero_tiles<VEC>(oz, ncols, ny, yt0, yt1, (int64_t)bx * 256);
         return;
@@ -921,7 +926,7 @@ __device__ inline void main_16_body(int bx, int z, int bz, const ZMeta* __restri
 #pragma unroll
             for (int it = 0; it < 4; ++it) acc[it] = mfma16<MODE>(ahi[it][s], h[s], acc[it]);
         }
-        tile_out<VEC>(T[yt & 1], acc, oz, ncols, ny, yt, (int64_t)bx * 256);
+        tile_out<VEC>(T[yt & 1], acc, oz, ncols, ny, yt, (int64_t)bx * 256, zm.acc != 0);
     };
     vec h0[S], l0[S], h1[S], l1[S];
     yload(yt0, h0, l0);
@@ -959,7 +964,7 @@ __global__ __launch_bounds__(256) void main_16(const ZMeta* __restrict__ zmeta, 
 
 template <int KMAX, bool VEC>
 static void launch_main(const scw_exec* ex, dim3 grid, hipStream_t st, int ytps, float* out, int zoff) {
-    const ZMeta* zm = ex->d_zmeta.as<ZMeta>();
+    const ZMeta* zm = ex->d_mzmeta.as<ZMeta>();  // main-GEMM entries [zoff, zoff + grid.y)
     const float2* U = ex->d_U.as<float2>();
     const int nx = (int)ex->nx, ny = (int)ex->ny;
     if (ex->dtype == SCW_DTYPE_FP32) {
@@ -1497,6 +1502,7 @@ struct FusedArgs {
     const ModalItem* mitems;
     const RegionDev* regions;
     const ZMeta* zmeta;
+    const ZMeta* mzmeta;  // main-GEMM entries (one ky chunk per region: one per z)
     const float2* Z;
     const uint32_t* zf;
     int64_t c_elems;
@@ -1629,7 +1635,7 @@ __global__ __launch_bounds__(256, 2) void fused16(const FusedArgs a) {
             default:  // U.x >> 8 consecutive column blocks from U.y of z-slice U.z, y split U.w
                 for (int t = 0; t < (U.x >> 8); ++t) {
                     if (t > 0) __syncthreads();  // the previous block's last transpose is read
-                    main_16_body<KMAX, VEC, MODE, true>(U.y + t, U.z, U.w, a.zmeta, a.regions, a.Cs, a.X,
+                    main_16_body<KMAX, VEC, MODE, true>(U.y + t, U.z, U.w, a.mzmeta, a.regions, a.Cs, a.X,
                                                         a.Yfrag, a.ylo_delta, a.nx, a.ny, a.ytiles_per_split, a.out,
                                                         lds);
                 }
@@ -1669,6 +1675,7 @@ static FusedArgs fused_args(const scw_exec* ex, float* out) {
     a.mitems = ex->d_mitems.as<ModalItem>();
     a.regions = ex->d_regions.as<RegionDev>();
     a.zmeta = ex->d_zmeta.as<ZMeta>();
+    a.mzmeta = ex->d_mzmeta.as<ZMeta>();
     a.Z = ex->d_Z.as<float2>();
     a.zf = ex->d_Zf.as<uint32_t>();
     a.c_elems = ex->c_elems;
@@ -1742,10 +1749,16 @@ int launch_gemm(const scw_exec* ex, float* out, hipStream_t st, int stage) {
     }
     if (ex->my_max > 0 && (stage == 0 || stage == 2)) {
         const dim3 g((unsigned)((ex->nx + 255) / 256), (unsigned)((ex->my_max + UB - 1) / UB), (unsigned)ex->nz);
-        if (ex->dtype == SCW_DTYPE_FP32)
-            hipLaunchKernelGGL(u_mfma, g, dim3(256), 0, st, ex->d_zmeta.as<ZMeta>(), ex->d_regions.as<RegionDev>(),
-                               ex->d_C.as<float2>(), ex->c_elems, ex->d_X.as<float2>(), (int)ex->nx,
-                               ex->d_U.as<float2>(), 0);
+        if (ex->dtype == SCW_DTYPE_FP32) {
+            if (ex->ka_max <= 128)
+                hipLaunchKernelGGL(u_mfma<130>, g, dim3(256), 0, st, ex->d_zmeta.as<ZMeta>(),
+                                   ex->d_regions.as<RegionDev>(), ex->d_C.as<float2>(), ex->c_elems,
+                                   ex->d_X.as<float2>(), (int)ex->nx, ex->d_U.as<float2>(), 0);
+            else
+                hipLaunchKernelGGL(u_mfma<258>, g, dim3(256), 0, st, ex->d_zmeta.as<ZMeta>(),
+                                   ex->d_regions.as<RegionDev>(), ex->d_C.as<float2>(), ex->c_elems,
+                                   ex->d_X.as<float2>(), (int)ex->nx, ex->d_U.as<float2>(), 0);
+        }
         else if (ex->ncblk > 0)
             hipLaunchKernelGGL(csum_kernel, dim3((unsigned)ex->ncblk), dim3(256), 0, st,
                                ex->d_zmeta.as<ZMeta>(), ex->d_regions.as<RegionDev>(), ex->d_C.as<float2>(),
@@ -1754,29 +1767,33 @@ int launch_gemm(const scw_exec* ex, float* out, hipStream_t st, int stage) {
     }
     if (stage != 0 && stage != 3) return SCW_OK;
     const int64_t ncols = 6 * ex->nx;
-    dim3 grid((unsigned)((ncols + 255) / 256), (unsigned)ex->nz, (unsigned)ex->ysplit);
     const int ytps = ex->ytiles_per_split;
     const bool vec = (ncols % 4) == 0;
+    // one launch per ky-chunk level: a level adds to what the previous wrote
+    for (size_t L = 0; L + 1 < ex->mlevel_off.size(); ++L) {
+        const int zoff = (int)ex->mlevel_off[L];
+        const dim3 grid((unsigned)((ncols + 255) / 256), (unsigned)(ex->mlevel_off[L + 1] - zoff), (unsigned)ex->ysplit);
 #define SCW_MAIN(K)                                                        \
-    if (vec) launch_main<K, true>(ex, grid, st, ytps, out, 0);             \
-    else launch_main<K, false>(ex, grid, st, ytps, out, 0);
-    if (ex->dtype == SCW_DTYPE_FP32) {
-        switch (ex->kmax) {
-            case 16: SCW_MAIN(16) break;
-            case 32: SCW_MAIN(32) break;
-            case 64: SCW_MAIN(64) break;
-            default: SCW_MAIN(128) break;
+    if (vec) launch_main<K, true>(ex, grid, st, ytps, out, zoff);          \
+    else launch_main<K, false>(ex, grid, st, ytps, out, zoff);
+        if (ex->dtype == SCW_DTYPE_FP32) {
+            switch (ex->kmax) {
+                case 16: SCW_MAIN(16) break;
+                case 32: SCW_MAIN(32) break;
+                case 64: SCW_MAIN(64) break;
+                default: SCW_MAIN(128) break;
+            }
+        } else {
+            switch (ex->kmax) {
+                case 32: SCW_MAIN(32) break;
+                case 64: SCW_MAIN(64) break;
+                case 96: SCW_MAIN(96) break;
+                default: SCW_MAIN(128) break;
+            }
         }
-    } else {
-        switch (ex->kmax) {
-            case 32: SCW_MAIN(32) break;
-            case 64: SCW_MAIN(64) break;
-            case 96: SCW_MAIN(96) break;
-            default: SCW_MAIN(128) break;
-        }
-    }
 #undef SCW_MAIN
-    SCW_HIP(hipGetLastError());
+        SCW_HIP(hipGetLastError());
+    }
     return SCW_OK;
 }
 

@@ -120,6 +120,12 @@ struct ZMeta {
     int64_t yfrag_off;  // element offset of the region's Y fragment table
     int64_t zrow_off;   // float2 offset of this z's row of exp(i kz_c (z - zref))
     int64_t c_off;      // float2 offset of C_z ([cells][3]) within one partial-sum slot
+    // main-GEMM entries (scw_exec::d_mzmeta): a region of more than 64 ky
+    // runs its main GEMM in chunks of ky rows, one entry per (z, chunk) with
+    // my, kp, u_off and yfrag_off of the chunk; entries of chunks after the
+    // first add to the output slice instead of writing it
+    int32_t zout;       // output z-slice
+    int32_t acc;        // 1: add to the output
 };
 
 // One wave-sized work item of the coefficient contraction.
@@ -196,6 +202,8 @@ struct scw_exec {
     std::vector<scw::ModalItem> mitems;   // grouped by row-tile class (MT = 1, 2, 4, 8)
     int64_t mclass_off[5] = {0, 0, 0, 0, 0};
     scw::DevBuf d_items, d_mitems, d_regions, d_zmeta, d_Z, d_Zf, d_X, d_C, d_U, d_Yfrag;
+    scw::DevBuf d_mzmeta;     // main-GEMM entries, by chunk level (level 0: one per z, in z order)
+    std::vector<int64_t> mlevel_off;  // first entry of each chunk level, then the entry count
     scw::DevBuf d_bfpx;       // FP16: shifted exponent planes of the dense regions, then the shifts
     scw::DevBuf d_cblk;       // 16-bit dtypes: slot-sum workgroups, int2 (z, first element of C_z)
     int64_t ncblk = 0;

@@ -144,8 +144,11 @@ static void group_values(const std::vector<cd>& v, double tol, std::vector<cd>& 
     for (size_t i = 0; i < n; ++i) idx[i] = grp[uid[i]];
 }
 
-static const int MY_MAX = 64;   // largest ky count of the GEMM form (K = 2*my <= 128)
-static const int MX_MAX = 128;  // largest kx count (LDS staging of the X contraction)
+// Largest ky count of the GEMM form: the main GEMM takes K = 2 my <= 128 per
+// launch, and a region of more ky runs it in chunks of at most 64 ky rows
+// whose launches after the first add to the output.
+static const int MY_MAX = 1024;
+static const int MX_MAX = 256;  // largest kx count (LDS staging of the FP32 X contraction)
 
 static void plan_region(RegionPlan& R, const scw_planewave64* W, const std::vector<double>& lns,
                         const std::vector<cd>& ehat, std::vector<float>& ellA, std::vector<int32_t>& ellC,
@@ -242,11 +245,11 @@ static void plan_region(RegionPlan& R, const scw_planewave64* W, const std::vect
     R.dense = dense;
 
     if (R.my > MY_MAX) {
-        R.why_not = "more than 64 distinct ky values";
+        R.why_not = "more than 1024 distinct ky values";
         return;
     }
     if (R.mx > MX_MAX) {
-        R.why_not = "more than 128 distinct kx values";
+        R.why_not = "more than 256 distinct kx values";
         return;
     }
     const int64_t slots = (int64_t)width * cells;

@@ -7,6 +7,8 @@ the field's max):
   GEMM FP32, BF16 x3   rel-L2 <= 1e-5 / 1e-4
   GEMM BF16, FP16      rel-L2 <= 1e-2 (reported precision forms, not the bar)
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -478,3 +480,30 @@ def test_auto_falls_back_to_direct_when_phase_overflows():
     assert ex.info()["form"] == "direct"
     ref = oracle.eval_grid_f64(t.W, t.z_layers, t.begin, t.end, xs, ys, zs)
     _check(ex.run_host(), ref, 1e-5, "thick evanescent slab, AUTO")
+
+
+@pytest.mark.parametrize("orders,dtype", [(65, "bf16x3"), (65, "fp32"), (81, "bf16x3"), (81, "fp16"), (81, "fp32"),
+                                          (129, "bf16x3"), (129, "bf16"), (129, "fp32")])
+def test_large_pupils_chunked_main_gemm(orders, dtype):
+    """81 x 81 and 129 x 129 orders: more than 64 distinct ky per region, so
+    the main GEMM runs over ky chunks of at most 64 rows (two and three
+    launches, each after the first adding to the output), and 129 kx take
+    FP32's wide X-contraction staging (more than 128 kx).  Against the oracle
+    on a grid across the interface, and the GEMM form is what AUTO picks.
+    65 orders: two chunks of 33 and 32 ky, on an odd nx (rows of 6 nx floats
+    that are not whole 16-byte groups, the element-wise accumulating
+    stores)."""
+    t = configs.layered_structure([1.0, 1.4 + 0.05j], [0.0], 1.0, 0.5, orders, ("x",)).unpack()
+    plan = Plan(t)
+    for r in (0, 1):
+        info = plan.region_info(r)
+        assert info["separable"] and info["mx"] == orders and info["my"] == orders
+    xs = configs.axis(-20, 20, 31 if orders == 65 else 32)
+    ys, zs = configs.axis(-18, 18, 24), configs.axis(-4, 3, 6)
+    ref = oracle.eval_grid_f64(t.W, t.z_layers, t.begin, t.end, xs, ys, zs, nthreads=min(16, os.cpu_count() or 1))
+    ex = Exec(plan, xs, ys, zs, dtype=dtype)
+    info = ex.info()
+    assert info["form"] == "gemm"
+    assert info["launches"] >= (orders + 63) // 64  # one main launch per ky chunk level
+    _check(ex.run_host(), ref, TOL[dtype], f"{orders}^2 pupil gemm {dtype}")
+
