@@ -104,6 +104,26 @@ def time_steps(ex, out, stream, steps, warmup, single="auto"):
     return res[way][0], res[way][1], way, {k: round(v[0], 4) for k, v in res.items()}
 
 
+def tuned_shape(ex, out, stream, key, args):
+    """Launch shape, z-run tiles and form of a GEMM-form exec from the
+    autotuner's table (scwatter_amd/tuning.json) under `key`, or tuned now
+    (and recorded there, and copied to --tuning-out) when the table has no
+    entry or --retune is given."""
+    import shutil
+
+    from scwatter_amd import autotune
+    cached = None if args.retune else autotune.apply_cached(ex, key)
+    if cached:
+        return {k: cached.get(k) for k in ("coef_items", "main_blocks", "modal_run", "coef_run", "single")} | {
+            "source": "table", "key": key}
+    entry, fam = autotune.tune(ex, out.ptr, stream, reps=4, key=key, path=autotune.TABLE)
+    if args.tuning_out:
+        shutil.copyfile(autotune.TABLE, args.tuning_out)
+    return dict(entry, source="tuned", key=key, candidates_ms={
+        name: {("/".join(str(x) for x in c) if isinstance(c, tuple) else str(c)): round(t, 4) for c, t in v.items()}
+        for name, v in fam.items()})
+
+
 def time_stages(ex, out, stream, reps):
     """Per-stage device time inside full runs: events between the stages of
     `reps` eager runs (coefficients, X contraction, MFMA GEMM), so every stage
@@ -257,6 +277,7 @@ def main():
     ap.add_argument("--single-launch", default="auto", choices=("auto", "on", "off"),
                     help="16-bit GEMM dtypes: also time the single-launch form and keep the faster (auto), "
                          "time only it (on), or only the stage-by-stage launches (off)")
+    ap.add_argument("--tuning-out", default=None, help="after tuning, also copy the tuning table to this path")
     ap.add_argument("--retune", action="store_true", help="time the launch-shape candidates even if the "
                     "tuning table has an entry for the run")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -267,7 +288,7 @@ def main():
               "(the evaluation would shard by z-slab, with no exchange between GPUs).", file=sys.stderr)
         sys.exit(2)
 
-    from scwatter_amd import Exec, Plan, configs, hip
+    from scwatter_amd import Exec, Plan, autotune, configs, hip
 
     if args.copy_calibration:
         n = 512 << 20
@@ -313,6 +334,8 @@ def main():
     for dt in dtypes:
         t0 = time.perf_counter()
         ex = Exec(plan, xs, ys, zs, form="gemm", dtype=dt)
+        tuned = tuned_shape(ex, out, stream, autotune.shape_key(ex, args.workload + ("p" if args.physical else "")),
+                            args)
         info = ex.info()
         ms, wall_ms, way, by_way = time_steps(ex, out, stream, args.steps, args.warmup, args.single_launch)
         samp = sample_device(out, shape, pidx)
@@ -323,7 +346,7 @@ def main():
             stages[name] = {"ms": round(sms[st - 1], 4), "flops": fl, "bytes": by}
         forms[f"gemm_{dt}"] = {"ms": ms, "wall_ms": wall_ms, "launch": way, "ms_by_launch": by_way,
                                "rate": info["units"] / (ms * 1e-3), "info": info,
-                               "stages": stages, "sample": samp}
+                               "stages": stages, "sample": samp, "tuned": tuned}
         log(f"[bench] gemm {dt}: {ms:.3f} ms/step ({way}; {by_way}; wall {wall_ms:.3f}), "
             f"{info['units'] / (ms * 1e-3):.3e} pts*waves/s, stages " +
             ", ".join(f"{k} {v['ms']:.3f} ms" for k, v in stages.items()) + f" ({time.perf_counter()-t0:.1f} s)")
@@ -392,8 +415,9 @@ def main():
         "forms": {k: {"ms_per_step": round(f["ms"], 4), **({"launch": f["launch"], "ms_by_launch": f["ms_by_launch"]}
                                                           if "launch" in f else {}),
                       "rate": f["rate"], "rel_l2": f["rel_l2"],
-                      "max_abs": f["max_abs"], **({"stages_ms": {s: v["ms"] for s, v in f["stages"].items()}}
-                                                  if "stages" in f else {})} for k, f in forms.items()},
+                      "max_abs": f["max_abs"], **({"stages_ms": {s: v["ms"] for s, v in f["stages"].items()},
+                                                   "tuned": f["tuned"]} if "stages" in f else {})}
+                  for k, f in forms.items()},
         "roofline": roof, "roofline_direct": roof_direct,
         "cpu_baseline": cpu_all, "cpu_baseline_single_thread": cpu_one,
     }
@@ -514,14 +538,7 @@ def bench_batch(args):
         t1 = time.perf_counter()
         bx = BatchExec(plans, xs, ys, [zs] * len(plans), dtype=dt)
         out = hip.DeviceArray(bx.shape, np.complex64)
-        key = autotune.shape_key(bx, "c4")
-        cached = None if args.retune else autotune.apply_cached(bx, key)
-        if cached:
-            tuned = {"coef_items": cached["coef_items"], "main_blocks": cached["main_blocks"], "source": "table"}
-        else:
-            best, cand = autotune.tune(bx, out.ptr, stream, reps=4, key=key, path=autotune.TABLE)
-            tuned = {"coef_items": best[0], "main_blocks": best[1], "source": "tuned",
-                     "candidates_ms": {f"{a}/{b}": round(v, 4) for (a, b), v in cand.items()}}
+        tuned = tuned_shape(bx, out, stream, autotune.shape_key(bx, "c4"), args)
         info = bx.info()
         ms, wall_ms, way, by_way = time_steps(bx, out, stream, args.steps, args.warmup, args.single_launch)
         sms = time_stages(bx, out, stream, 3)

@@ -149,6 +149,12 @@ int scw_batch_exec_create(int n, scw_plan* const* plans, const double* xs, size_
  * and graph launch of the exec returns SCW_ERR_ARG until a later call
  * succeeds; the exec can always be destroyed. */
 int scw_exec_set_launch(scw_exec* exec, int64_t coef_items, int64_t main_blocks);
+/* Tile shape of the GEMM form's coefficient contraction: the z runs of one
+ * work item, at most modal_run slices for the MFMA mode contraction (1..64;
+ * its row-tile class is 1, 2, 4 or 8 tiles of 8 z) and at most coef_run for
+ * the VALU term contraction (1..8); 0 keeps the default (64, 8).  Rebuilds
+ * and fails like scw_exec_set_launch.  Used by the autotuner. */
+int scw_exec_set_tiles(scw_exec* exec, int modal_run, int coef_run);
 /* Single-launch form of a GEMM-form exec with a 16-bit dtype (no
  * counterpart in the reference, whose gpu_cw_evaluate launches one kernel per
  * plane, gpuEvaluator.cu:149-157): with on != 0, every later run (and graph

@@ -22,7 +22,7 @@ EXPORTS = (
     "scw_last_error", "scw_abi_version", "scw_device_count", "scw_device_mem_info",
     "scw_plan_create", "scw_plan_destroy", "scw_plan_layers", "scw_plan_region_info", "scw_plan_device_bytes",
     "scw_exec_create", "scw_exec_run", "scw_exec_run_stage", "scw_exec_run_host", "scw_exec_destroy", "scw_exec_info",
-    "scw_exec_stage_info", "scw_batch_exec_create", "scw_exec_set_launch",
+    "scw_exec_stage_info", "scw_batch_exec_create", "scw_exec_set_launch", "scw_exec_set_tiles",
     "scw_exec_graph_capture", "scw_exec_graph_launch", "scw_exec_set_persistent", "scw_exec_persistent_info",
     "scw_exec_check", "scw_test_exec_stall",
     "scw_cw_load", "scw_cw_save", "scw_cw_create", "scw_cw_destroy", "scw_cw_set_incident", "scw_cw_add_layer",
@@ -79,6 +79,7 @@ def lib():
                                                  ctypes.c_size_t, ctypes.POINTER(_dp), _szp, ctypes.c_double,
                                                  ctypes.c_int, ctypes.POINTER(_vp)]),
         "scw_exec_set_launch": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64]),
+        "scw_exec_set_tiles": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
         "scw_exec_graph_capture": (ctypes.c_int, [_vp, _vp, _vp]),
         "scw_exec_graph_launch": (ctypes.c_int, [_vp, _vp]),
         "scw_exec_set_persistent": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int64, ctypes.c_double]),

@@ -436,7 +436,10 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
         int zend = z + 1;  // maximal stretch of consecutive z in region v
         while (zend < nz && zv[zend] == v) ++zend;
         const bool modal = rd[v].dense && rd[v].width > 0 && zend - z >= modal_min;
-        const int cap = modal ? 64 : 8;
+        // z-run tile of a work item: up to 64 z for the mode contraction
+        // (8 row tiles of 8 z), up to 8 z for the VALU contraction
+        const int cap = modal ? (ex->tune_modal_run > 0 ? (int)ex->tune_modal_run : 64)
+                              : (ex->tune_coef_run > 0 ? (int)ex->tune_coef_run : 8);
         const int nrun = (zend - z + cap - 1) / cap;  // balanced runs
         for (int k = 0; k < nrun; ++k)
             runs.push_back({z + (int)((int64_t)(zend - z) * k / nrun), z + (int)((int64_t)(zend - z) * (k + 1) / nrun),
@@ -998,10 +1001,9 @@ int scw_batch_exec_create(int n, scw_plan* const* plans, const double* xs, size_
     return SCW_OK;
 }
 
-int scw_exec_set_launch(scw_exec* ex, int64_t coef_items, int64_t main_blocks) {
-    if (!ex) return fail(SCW_ERR_ARG, "exec is NULL");
-    if (coef_items < 0 || main_blocks < 0) return fail(SCW_ERR_ARG, "negative launch parameter");
-    if (ex->form != SCW_FORM_GEMM) return SCW_OK;  // the direct form has no tunable launch shape
+// Rebuilds the GEMM-form work lists and tables after a launch-shape or tile
+// change (the grid and plans are those of the exec).
+static int rebuild_gemm(scw_exec* ex) {
     DeviceGuard g(ex->plan->device);
     if (g.st != SCW_OK) return g.st;
     if (ex->graph_exec) {
@@ -1012,8 +1014,6 @@ int scw_exec_set_launch(scw_exec* ex, int64_t coef_items, int64_t main_blocks) {
         (void)hipGraphDestroy(ex->graph);
         ex->graph = nullptr;
     }
-    ex->tune_coef_items = coef_items;
-    ex->tune_main_blocks = main_blocks;
     ex->nslots_max = 1;
     // prep_gemm replaces the device tables one by one; a failure part way
     // leaves some of them released next to rebuilt work lists, so the exec
@@ -1027,6 +1027,25 @@ int scw_exec_set_launch(scw_exec* ex, int64_t coef_items, int64_t main_blocks) {
     }
     if (st == SCW_OK) ex->invalid = false;
     return st;
+}
+
+int scw_exec_set_launch(scw_exec* ex, int64_t coef_items, int64_t main_blocks) {
+    if (!ex) return fail(SCW_ERR_ARG, "exec is NULL");
+    if (coef_items < 0 || main_blocks < 0) return fail(SCW_ERR_ARG, "negative launch parameter");
+    if (ex->form != SCW_FORM_GEMM) return SCW_OK;  // the direct form has no tunable launch shape
+    ex->tune_coef_items = coef_items;
+    ex->tune_main_blocks = main_blocks;
+    return rebuild_gemm(ex);
+}
+
+int scw_exec_set_tiles(scw_exec* ex, int modal_run, int coef_run) {
+    if (!ex) return fail(SCW_ERR_ARG, "exec is NULL");
+    if (modal_run < 0 || modal_run > 64 || coef_run < 0 || coef_run > 8)
+        return fail(SCW_ERR_ARG, "tile: modal_run in [0, 64], coef_run in [0, 8] (0 = default)");
+    if (ex->form != SCW_FORM_GEMM) return SCW_OK;
+    ex->tune_modal_run = modal_run;
+    ex->tune_coef_run = coef_run;
+    return rebuild_gemm(ex);
 }
 
 static int refuse_invalid(const scw_exec* ex);

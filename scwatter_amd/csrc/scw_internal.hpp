@@ -209,6 +209,8 @@ struct scw_exec {
     int64_t ncblk = 0;
     int64_t tune_coef_items = 0;   // launch-shape choices (0 = default)
     int64_t tune_main_blocks = 0;
+    int64_t tune_modal_run = 0;    // z-run tiles (0 = default: 64 mode-contraction, 8 VALU)
+    int64_t tune_coef_run = 0;
     std::vector<double> hx, hy;    // grid kept for re-planning launch shapes
     std::vector<scw_plan*> batch;  // plans of a batch exec (empty for a single plan)
     std::vector<double> hz;        // z of every output slice

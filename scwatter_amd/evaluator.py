@@ -121,6 +121,10 @@ class Exec:
         """GEMM-form launch shape (0 = default); rebuilds the work lists."""
         check(_native.lib().scw_exec_set_launch(self.handle, int(coef_items), int(main_blocks)))
 
+    def set_tiles(self, modal_run=0, coef_run=0):
+        """GEMM-form z-run tiles of the coefficient contraction (0 = default 64 / 8)."""
+        check(_native.lib().scw_exec_set_tiles(self.handle, int(modal_run), int(coef_run)))
+
     def set_persistent(self, on=True, workgroups=0, wait_ms=0.0):
         """Single-launch form (GEMM form, 16-bit dtypes): every run is one
         persistent kernel; workgroups 0 = resident capacity, wait_ms 0 = 10 s."""

@@ -507,3 +507,20 @@ def test_large_pupils_chunked_main_gemm(orders, dtype):
     assert info["launches"] >= (orders + 63) // 64  # one main launch per ky chunk level
     _check(ex.run_host(), ref, TOL[dtype], f"{orders}^2 pupil gemm {dtype}")
 
+
+
+@pytest.mark.parametrize("dtype", ["bf16x3", "fp32"])
+def test_tiles_change_only_the_schedule(tile_small, dtype):
+    """z-run tiles (the autotuner's second family): mode-contraction runs of
+    32 and 16 slices and VALU runs of 4 give the default field (the same
+    sums, in other work items) and match the oracle."""
+    plan, g, ref = tile_small
+    ex = Exec(plan, *g, form="gemm", dtype=dtype)
+    E0 = ex.run_host()
+    for modal_run, coef_run in ((32, 0), (16, 4), (0, 4), (8, 1)):
+        ex.set_tiles(modal_run, coef_run)
+        E = ex.run_host()
+        assert oracle.rel_l2(E, E0) < 1e-6, (modal_run, coef_run)
+        _check(E, ref, TOL[dtype], f"tile sample tiles {modal_run}/{coef_run} {dtype}")
+    with pytest.raises(Exception, match="ARG"):
+        ex.set_tiles(65, 0)
