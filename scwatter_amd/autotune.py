@@ -38,19 +38,21 @@ def load_table(path=TABLE):
     return {}
 
 
-def _apply(ex, e):
+def _apply(ex, e, single=True):
     ex.set_launch(e["coef_items"], e["main_blocks"])
     if e.get("modal_run", 0) or e.get("coef_run", 0):
         ex.set_tiles(e.get("modal_run", 0), e.get("coef_run", 0))
-    if e.get("single"):
+    if single and e.get("single"):
         ex.set_persistent(True)
 
 
-def apply_cached(ex, key, path=TABLE):
-    """Apply the recorded shape for `key`; returns its table entry or None."""
+def apply_cached(ex, key, path=TABLE, single=True):
+    """Apply the recorded shape for `key` (with single=False, the launch shape
+    and tiles only, keeping the stage-by-stage form); returns its table entry
+    or None."""
     e = load_table(path).get(key)
     if e:
-        _apply(ex, e)
+        _apply(ex, e, single)
     return e
 
 

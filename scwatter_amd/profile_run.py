@@ -3,8 +3,8 @@
     python -m scwatter_amd.profile_run --workload c4 --dtypes bf16x3,fp32 --steps 5 --direct
 
 builds the workload bench.py measures (same structures, grid and launch shapes:
-the autotuner's table entry when there is one), runs one warm-up and --steps
-evaluations of the GEMM form per dtype, and with --direct the direct form
+the autotuner's table entry when there is one, in the stage-by-stage form),
+runs one warm-up and --steps evaluations of the GEMM form per dtype, and with --direct the direct form
 (for the batched sweep c4: one z-slice of the first instance of each order
 class, as bench.py times it).  It checks nothing and times nothing itself;
 it is the program to put after `rocprofv3 ... --`, where each counter pass
@@ -21,6 +21,10 @@ from . import BatchExec, Exec, Plan, autotune, configs, hip, lib
 
 def _log(msg):
     print(f"[profile_run] {msg}", flush=True)
+
+
+def _shape(e):
+    return {k: e[k] for k in ("coef_items", "main_blocks", "modal_run", "coef_run") if k in e} if e else "default"
 
 
 def _steps(ex, out, n, what):
@@ -40,9 +44,9 @@ def run_batch(args):
     _log(f"c4: {len(plans)} instances planned ({time.perf_counter() - t0:.1f} s)")
     for dt in args.dtypes:
         bx = BatchExec(plans, xs, ys, [zs] * len(plans), dtype=dt)
-        shape = autotune.apply_cached(bx, autotune.shape_key(bx, "c4"))
+        shape = autotune.apply_cached(bx, autotune.shape_key(bx, "c4"), single=False)
         out = hip.DeviceArray(bx.shape, np.complex64)
-        _steps(bx, out, args.steps, f"c4 gemm {dt} (launch shape {shape or 'default'})")
+        _steps(bx, out, args.steps, f"c4 gemm {dt} (launch shape {_shape(shape)})")
         bx.destroy()
         out.free()
     if args.direct:
@@ -69,7 +73,8 @@ def run_single(args):
     out = hip.DeviceArray((zs.size, ys.size, xs.size, 3), np.complex64)
     for dt in args.dtypes:
         ex = Exec(plan, xs, ys, zs, form="gemm", dtype=dt)
-        _steps(ex, out, args.steps, f"{args.workload} gemm {dt}")
+        shape = autotune.apply_cached(ex, autotune.shape_key(ex, args.workload), single=False)
+        _steps(ex, out, args.steps, f"{args.workload} gemm {dt} (launch shape {_shape(shape)})")
         ex.destroy()
     if args.direct:
         ex = Exec(plan, xs, ys, zs, form="direct")
