@@ -252,6 +252,12 @@ void scw_test_fail_alloc_after(int64_t k);
  * expire (a host-side condition: every kernel still ends normally); on = 0
  * restores the counts.  Not for production use. */
 int scw_test_exec_stall(scw_exec* exec, int on);
+/* The host half of scw_plan_create (the same validation and FP64 region
+ * planning, no device): per region r, info[8 r ..] = separable, dense, mx,
+ * my, mc, width, n_terms, waves.  For tests of the planner without a GPU
+ * (the host sanitizer build).  Not for production use. */
+int scw_test_plan_host(const scw_planewave64* waves, size_t n_waves, const double* z_layers, int layers,
+                       const int32_t* waves_begin, const int32_t* waves_end, int64_t* info);
 
 /* ---- .cw files (CoupledWaveStructure.h:7-134) --------------------------- */
 int scw_cw_load(const char* path, scw_cw** cw);

@@ -319,18 +319,18 @@ int scw_device_mem_info(int device, size_t* free_bytes, size_t* total_bytes) {
     return SCW_OK;
 }
 
-int scw_plan_create(int device, const scw_planewave64* waves, size_t n_waves, const double* z_layers,
-                    int layers, const int32_t* waves_begin, const int32_t* waves_end, scw_plan** out) {
-    if (!out) return fail(SCW_ERR_ARG, "plan output is NULL");
-    *out = nullptr;
+}  // extern "C"
+
+namespace {
+
+// Argument checks of scw_plan_create that need no device.
+int validate_tables(const scw_planewave64* waves, size_t n_waves, const double* z_layers, int layers,
+                    const int32_t* waves_begin, const int32_t* waves_end) {
     if (layers < 0) return fail(SCW_ERR_ARG, "layers < 0");
     if (n_waves > 0 && !waves) return fail(SCW_ERR_ARG, "waves is NULL");
     if (layers > 0 && !z_layers) return fail(SCW_ERR_ARG, "z_layers is NULL");
     if (!waves_begin || !waves_end) return fail(SCW_ERR_ARG, "waves_begin / waves_end is NULL");
     if (n_waves > (size_t)INT32_MAX) return fail(SCW_ERR_ARG, "more than 2^31-1 waves");
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(SCW_ERR_DEVICE, "no HIP device");
-    if (device < 0 || device >= ndev) return fail(SCW_ERR_ARG, "device %d out of range [0, %d)", device, ndev);
     for (int l = 0; l < layers; ++l)
         if (!std::isfinite(z_layers[l])) return fail(SCW_ERR_ARG, "z_layers[%d] is not finite", l);
     for (int r = 0; r <= layers; ++r) {
@@ -343,13 +343,23 @@ int scw_plan_create(int device, const scw_planewave64* waves, size_t n_waves, co
         for (int j = 0; j < 12; ++j)
             if (!std::isfinite(v[j])) return fail(SCW_ERR_ARG, "wave %zu has a non-finite value", i);
     }
+    return SCW_OK;
+}
 
-    DeviceGuard g(device);
-    if (g.st != SCW_OK) return g.st;
+// Host tables of a plan, before their upload.
+struct HostTables {
+    std::vector<double> k64, e64, lns;
+    std::vector<float> kf, ellA;
+    std::vector<int32_t> ellC;
+    std::vector<unsigned char> dreg;
+};
 
-    scw_plan* P = new (std::nothrow) scw_plan();
-    if (!P) return fail(SCW_ERR_NOMEM, "host allocation failed");
-    P->device = device;
+// The host half of planning (FP64): the direct form's tables, every
+// region's separable decomposition and the GEMM form's amplitude tables.
+// Fills P's host fields (layers, boundaries, regions) and H; touches no
+// device.
+int plan_host(scw_plan* P, const scw_planewave64* waves, size_t n_waves, const double* z_layers, int layers,
+              const int32_t* waves_begin, const int32_t* waves_end, HostTables& H) {
     P->layers = layers;
     P->n_waves = (int64_t)n_waves;
     P->zl.assign(z_layers, z_layers + layers);
@@ -358,8 +368,14 @@ int scw_plan_create(int device, const scw_planewave64* waves, size_t n_waves, co
     // Direct-form tables.
     const double log2e = 1.4426950408889634;
     const double inv2pi = 0.15915494309189535;
-    std::vector<double> k64(6 * n_waves), e64(6 * n_waves), lns(n_waves);
-    std::vector<float> kf(4 * n_waves);
+    std::vector<double>& k64 = H.k64;
+    std::vector<double>& e64 = H.e64;
+    std::vector<double>& lns = H.lns;
+    std::vector<float>& kf = H.kf;
+    k64.assign(6 * n_waves, 0.0);
+    e64.assign(6 * n_waves, 0.0);
+    lns.assign(n_waves, 0.0);
+    kf.assign(4 * n_waves, 0.0f);
     std::vector<cd> ehat(3 * n_waves);
     for (size_t i = 0; i < n_waves; ++i) {
         const scw_planewave64& w = waves[i];
@@ -380,8 +396,6 @@ int scw_plan_create(int device, const scw_planewave64* waves, size_t n_waves, co
     }
 
     // Region planning.
-    std::vector<float> ellA;
-    std::vector<int32_t> ellC;
     int64_t slotsA = 0, slotsC = 0;
     for (int r = 0; r <= layers; ++r) {
         RegionPlan& R = P->regions[r];
@@ -394,20 +408,54 @@ int scw_plan_create(int device, const scw_planewave64* waves, size_t n_waves, co
         R.real_transverse = true;
         for (int64_t i = R.wbegin; i < R.wend; ++i)
             if (waves[i].k[0][1] != 0.0 || waves[i].k[1][1] != 0.0) R.real_transverse = false;
-        try {
-            plan_region(R, waves, lns, ehat, ellA, ellC, slotsA, slotsC);
-        } catch (const std::bad_alloc&) {
-            delete P;
-            return fail(SCW_ERR_NOMEM, "host allocation failed while planning region %d", r);
-        }
+        plan_region(R, waves, lns, ehat, H.ellA, H.ellC, slotsA, slotsC);
     }
     P->ell_elems = slotsA;
-    std::vector<unsigned char> dreg(direct_region_bytes() * (layers + 1));
+    H.dreg.assign(direct_region_bytes() * (layers + 1), 0);
     for (int r = 0; r <= layers; ++r)
-        direct_region_fill(&dreg[direct_region_bytes() * r], P->regions[r].wbegin, P->regions[r].wend,
+        direct_region_fill(&H.dreg[direct_region_bytes() * r], P->regions[r].wbegin, P->regions[r].wend,
                            P->regions[r].real_transverse ? 1 : 0);
+    return SCW_OK;
+}
 
-    int st = SCW_OK;
+}  // namespace
+
+extern "C" {
+
+int scw_plan_create(int device, const scw_planewave64* waves, size_t n_waves, const double* z_layers,
+                    int layers, const int32_t* waves_begin, const int32_t* waves_end, scw_plan** out) {
+    if (!out) return fail(SCW_ERR_ARG, "plan output is NULL");
+    *out = nullptr;
+    int st = validate_tables(waves, n_waves, z_layers, layers, waves_begin, waves_end);
+    if (st != SCW_OK) return st;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(SCW_ERR_DEVICE, "no HIP device");
+    if (device < 0 || device >= ndev) return fail(SCW_ERR_ARG, "device %d out of range [0, %d)", device, ndev);
+
+    DeviceGuard g(device);
+    if (g.st != SCW_OK) return g.st;
+
+    scw_plan* P = new (std::nothrow) scw_plan();
+    if (!P) return fail(SCW_ERR_NOMEM, "host allocation failed");
+    P->device = device;
+    HostTables H;
+    try {
+        st = plan_host(P, waves, n_waves, z_layers, layers, waves_begin, waves_end, H);
+    } catch (const std::bad_alloc&) {
+        delete P;
+        return fail(SCW_ERR_NOMEM, "host allocation failed while planning");
+    }
+    if (st != SCW_OK) {
+        delete P;
+        return st;
+    }
+    const std::vector<double>& k64 = H.k64;
+    const std::vector<double>& e64 = H.e64;
+    const std::vector<double>& lns = H.lns;
+    const std::vector<float>& kf = H.kf;
+    const std::vector<float>& ellA = H.ellA;
+    const std::vector<int32_t>& ellC = H.ellC;
+    const std::vector<unsigned char>& dreg = H.dreg;
     if ((st = P->d_k64.upload(k64.data(), k64.size() * sizeof(double))) != SCW_OK ||
         (st = P->d_e64.upload(e64.data(), e64.size() * sizeof(double))) != SCW_OK ||
         (st = P->d_lns.upload(lns.data(), lns.size() * sizeof(double))) != SCW_OK ||
@@ -468,6 +516,34 @@ int scw_plan_device_bytes(const scw_plan* plan, size_t* bytes) {
     if (!plan || !bytes) return fail(SCW_ERR_ARG, "NULL argument");
     *bytes = plan->d_k64.bytes + plan->d_e64.bytes + plan->d_lns.bytes + plan->d_kf.bytes +
              plan->d_ellA.bytes + plan->d_ellC.bytes + plan->d_bfp.bytes + plan->d_dreg.bytes + plan->d_zl.bytes;
+    return SCW_OK;
+}
+
+int scw_test_plan_host(const scw_planewave64* waves, size_t n_waves, const double* z_layers, int layers,
+                       const int32_t* waves_begin, const int32_t* waves_end, int64_t* info) {
+    if (!info) return fail(SCW_ERR_ARG, "info is NULL");
+    int st = validate_tables(waves, n_waves, z_layers, layers, waves_begin, waves_end);
+    if (st != SCW_OK) return st;
+    scw_plan P;  // host fields only: its device buffers stay empty
+    HostTables H;
+    try {
+        st = plan_host(&P, waves, n_waves, z_layers, layers, waves_begin, waves_end, H);
+    } catch (const std::bad_alloc&) {
+        return fail(SCW_ERR_NOMEM, "host allocation failed while planning");
+    }
+    if (st != SCW_OK) return st;
+    for (int r = 0; r <= layers; ++r) {
+        const RegionPlan& R = P.regions[r];
+        int64_t* o = info + 8 * r;
+        o[0] = R.separable ? 1 : 0;
+        o[1] = R.dense ? 1 : 0;
+        o[2] = R.mx;
+        o[3] = R.my;
+        o[4] = R.mc;
+        o[5] = R.width;
+        o[6] = R.n_terms;
+        o[7] = R.wend - R.wbegin;
+    }
     return SCW_OK;
 }
 
