@@ -195,6 +195,7 @@ int launch_direct_raw(const scw_plan* P, const void* region_table, const int32_t
 }
 
 int launch_direct(const scw_exec* ex, float* out, hipStream_t st) {
+    TraceRange tr("direct form");
     return launch_direct_raw(ex->plan, ex->plan->d_dreg.p, ex->d_zslab.as<int32_t>(),
                              ex->d_z.as<double>(), ex->d_dx.as<float>(), ex->d_dy.as<float>(), ex->nx, ex->ny, ex->nz,
                              ex->xc, ex->yc, ex->direct_ppt, ex->direct_split, ex->d_C.as<float>(), out, st);

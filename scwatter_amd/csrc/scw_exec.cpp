@@ -937,6 +937,7 @@ int scw_exec_create(scw_plan* plan, const double* xs, size_t nx, const double* y
                     size_t nz, double region_tol, int form, int dtype, scw_exec** out) {
     if (!out) return fail(SCW_ERR_ARG, "exec output is NULL");
     *out = nullptr;
+    TraceRange tr("scw_exec_create");
     scw_exec* ex = new (std::nothrow) scw_exec();
     if (!ex) return fail(SCW_ERR_NOMEM, "host allocation failed");
     const int st = exec_init(ex, plan, xs, nx, ys, ny, zs, nz, region_tol, form, dtype);
@@ -972,6 +973,7 @@ int scw_batch_exec_create(int n, scw_plan* const* plans, const double* xs, size_
     for (size_t i = 0; i < ny; ++i)
         if (!std::isfinite(ys[i])) return fail(SCW_ERR_ARG, "ys[%zu] is not finite", i);
 
+    TraceRange tr("scw_batch_exec_create");
     scw_exec* ex = new (std::nothrow) scw_exec();
     if (!ex) return fail(SCW_ERR_NOMEM, "host allocation failed");
     int st = SCW_OK;
@@ -1021,6 +1023,7 @@ int scw_batch_exec_create(int n, scw_plan* const* plans, const double* xs, size_
 // Rebuilds the GEMM-form work lists and tables after a launch-shape or tile
 // change (the grid and plans are those of the exec).
 static int rebuild_gemm(scw_exec* ex) {
+    TraceRange tr("scw_exec rebuild");
     DeviceGuard g(ex->plan->device);
     if (g.st != SCW_OK) return g.st;
     if (ex->graph_exec) {
@@ -1158,6 +1161,7 @@ int scw_exec_run(scw_exec* ex, void* out_device, void* stream) {
     if (!ex) return fail(SCW_ERR_ARG, "exec is NULL");
     if (!out_device) return fail(SCW_ERR_ARG, "output pointer is NULL");
     if (ex->invalid) return refuse_invalid(ex);
+    TraceRange tr(ex->form == SCW_FORM_DIRECT ? "scw_exec_run direct" : "scw_exec_run gemm");
     DeviceGuard g(ex->plan->device);
     if (g.st != SCW_OK) return g.st;
     hipStream_t s = static_cast<hipStream_t>(stream);

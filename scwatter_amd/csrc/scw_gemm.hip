@@ -1729,10 +1729,14 @@ int fused_launch(const scw_exec* ex, float* out, hipStream_t st, int* per_cu) {
 // Stages: 1 = coefficient contraction (VALU ELL items and MFMA mode items),
 // 2 = X contraction, 3 = main MFMA GEMM; 0 = all three.
 int launch_gemm(const scw_exec* ex, float* out, hipStream_t st, int stage) {
-    if (stage == 0 && ex->persistent) return fused_launch(ex, out, st, nullptr);
+    if (stage == 0 && ex->persistent) {
+        TraceRange tr("gemm single launch");
+        return fused_launch(ex, out, st, nullptr);
+    }
     const int nitems = (int)ex->items.size();
     const int nmitems = (int)ex->mitems.size();
     if (stage == 0 || stage == 1) {
+        TraceRange tr("gemm stage 1: coefficient contraction");
         if (nitems > 0) {
             hipLaunchKernelGGL(coef_kernel, dim3((unsigned)((nitems + 3) / 4)), dim3(256), 0, st,
                                ex->d_items.as<CoefItem>(), nitems, ex->d_regions.as<RegionDev>(),
@@ -1748,6 +1752,7 @@ int launch_gemm(const scw_exec* ex, float* out, hipStream_t st, int stage) {
         }
     }
     if (ex->my_max > 0 && (stage == 0 || stage == 2)) {
+        TraceRange tr("gemm stage 2: X contraction / slot sum");
         const dim3 g((unsigned)((ex->nx + 255) / 256), (unsigned)((ex->my_max + UB - 1) / UB), (unsigned)ex->nz);
         if (ex->dtype == SCW_DTYPE_FP32) {
             if (ex->ka_max <= 128)
@@ -1766,6 +1771,7 @@ int launch_gemm(const scw_exec* ex, float* out, hipStream_t st, int stage) {
         SCW_HIP(hipGetLastError());
     }
     if (stage != 0 && stage != 3) return SCW_OK;
+    TraceRange tr("gemm stage 3: main GEMM");
     const int64_t ncols = 6 * ex->nx;
     const int ytps = ex->ytiles_per_split;
     const bool vec = (ncols % 4) == 0;

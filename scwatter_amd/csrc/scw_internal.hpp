@@ -3,6 +3,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <complex>
 #include <cstdint>
@@ -24,6 +25,16 @@ int hip_fail(hipError_t e, const char* what);
         hipError_t e_ = (call);                                  \
         if (e_ != hipSuccess) return ::scw::hip_fail(e_, #call); \
     } while (0)
+
+// roctx range over a scope (plan and exec builds, runs, stage launches):
+// what `rocprofv3 --marker-trace` shows of the library.  Without a profiler
+// attached a push / pop is a call that returns at once.
+struct TraceRange {
+    explicit TraceRange(const char* name) { roctxRangePushA(name); }
+    ~TraceRange() { roctxRangePop(); }
+    TraceRange(const TraceRange&) = delete;
+    TraceRange& operator=(const TraceRange&) = delete;
+};
 
 // Makes `dev` current for the scope and restores the caller's device after.
 struct DeviceGuard {

@@ -432,6 +432,7 @@ int scw_plan_create(int device, const scw_planewave64* waves, size_t n_waves, co
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(SCW_ERR_DEVICE, "no HIP device");
     if (device < 0 || device >= ndev) return fail(SCW_ERR_ARG, "device %d out of range [0, %d)", device, ndev);
 
+    TraceRange tr("scw_plan_create");
     DeviceGuard g(device);
     if (g.st != SCW_OK) return g.st;
 

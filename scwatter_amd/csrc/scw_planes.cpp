@@ -75,6 +75,7 @@ using namespace scw;
 extern "C" int scw_eval_planes(scw_plan* plan, float x_start, float y_start, float z_start, float x, float y, float z,
                                float d, int N, double region_tol, int form, int dtype, void* E_xy, void* E_xz,
                                void* E_yz, void* stream) {
+    TraceRange tr("scw_eval_planes");
     if (!plan) return fail(SCW_ERR_ARG, "plan is NULL");
     if (N < 1 || N > (1 << 16)) return fail(SCW_ERR_ARG, "N = %d out of range [1, 65536]", N);
     if (!std::isfinite(x_start) || !std::isfinite(y_start) || !std::isfinite(z_start) || !std::isfinite(x) ||
