@@ -5,6 +5,7 @@ import fails loudly — there is no CPU fallback.
 """
 import ctypes
 import os
+import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libscwatter_amd.so")
@@ -39,6 +40,7 @@ class ScwError(RuntimeError):
 
 
 _lib = None
+_lib_lock = threading.Lock()
 _vp = ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
@@ -48,7 +50,16 @@ _intp = ctypes.POINTER(ctypes.c_int)
 
 
 def lib():
-    """Load libscwatter_amd.so (raises OSError if it has not been built)."""
+    """Load libscwatter_amd.so (raises OSError if it has not been built).
+    Thread-safe: plans are created from thread pools, and the first caller
+    loads the library while the others wait."""
+    if _lib is not None:
+        return _lib
+    with _lib_lock:
+        return _load()
+
+
+def _load():
     global _lib
     if _lib is not None:
         return _lib

@@ -35,6 +35,7 @@ int g_dtype = SCW_DTYPE_FP32;
 
 int ensure_plan() {
     if (g_plan) return SCW_OK;
+    scw::hip_runtime_once();
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) dev = 0;
     std::vector<double> zl(z_layers.begin(), z_layers.end());  // the float boundaries the reference compares with
@@ -141,6 +142,7 @@ void gpu_cw_evaluate(std::complex<float>* E_xy, std::complex<float>* E_xz, std::
     }
     if (!g_plan) {
         int cur = -1;
+        scw::hip_runtime_once();
         (void)hipGetDevice(&cur);
         if (hipSetDevice(device) != hipSuccess) {
             g_status = scw::fail(SCW_ERR_ARG, "gpu_cw_evaluate: device %d unavailable", device);

@@ -37,10 +37,16 @@ struct TraceRange {
 };
 
 // Makes `dev` current for the scope and restores the caller's device after.
+// Initialises the HIP runtime once per process, on the first thread that asks
+// (the others wait): the runtime's lazy initialisation is not safe to enter
+// from several threads at once, and callers create plans from thread pools.
+void hip_runtime_once();
+
 struct DeviceGuard {
     int prev = -1;
     int st = 0;
     explicit DeviceGuard(int dev) {
+        hip_runtime_once();
         if (hipGetDevice(&prev) != hipSuccess) prev = -1;
         hipError_t e = hipSetDevice(dev);
         if (e != hipSuccess) st = hip_fail(e, "hipSetDevice");

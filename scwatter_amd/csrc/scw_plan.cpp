@@ -14,6 +14,7 @@
 #include <cstring>
 #include <array>
 #include <atomic>
+#include <mutex>
 #include <unordered_map>
 #include <new>
 
@@ -36,6 +37,14 @@ int fail(int status, const char* fmt, ...) {
     vsnprintf(g_err, sizeof(g_err), fmt, ap);
     va_end(ap);
     return status;
+}
+
+void hip_runtime_once() {
+    static std::once_flag once;
+    std::call_once(once, [] {
+        int n = 0;
+        (void)hipGetDeviceCount(&n);
+    });
 }
 
 int hip_fail(hipError_t e, const char* what) {
@@ -301,6 +310,7 @@ int scw_abi_version(void) { return SCW_ABI_VERSION; }
 
 int scw_device_count(int* count) {
     if (!count) return fail(SCW_ERR_ARG, "count is NULL");
+    hip_runtime_once();
     int n = 0;
     hipError_t e = hipGetDeviceCount(&n);
     if (e != hipSuccess) {
