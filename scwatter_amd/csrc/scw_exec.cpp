@@ -141,6 +141,7 @@ static int build_units(scw_exec* ex) {
         if (k == 3 && (end - off) % 4 != 0 && ex->dtype != SCW_DTYPE_FP32)
             return fail(SCW_ERR_ARG, "internal: 64-z mode items not in whole workgroups");
         for (int i = off; i < end; i += 4) {
+            if (ex->mitems[i].nz == 0) continue;  // an empty workgroup of the XCD-grouped order
             int z0 = ex->mitems[i].z0;
             for (int w = 1; w < 4 && i + w < end; ++w) z0 = std::min(z0, ex->mitems[i + w].z0);
             cu.push_back({z0, make_int4(k < 3 ? kModal1 + k : kModalWg, i, end, 0)});
@@ -728,12 +729,54 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
             // long ones set the end of the launch while most CUs idle; taken
             // first, the short ones fill the end.  64-slab sweep: mode
             // contraction 1.14 -> 1.04 ms (BF16x3), 0.99 -> 0.80 ms (FP16).
-            std::vector<size_t> gi(wg.size() / 4);
-            for (size_t q = 0; q < gi.size(); ++q) gi[q] = q;
-            std::stable_sort(gi.begin(), gi.end(), [&](size_t a, size_t b) {
-                return wg[4 * a].c1 - wg[4 * a].c0 > wg[4 * b].c1 - wg[4 * b].c0;
-            });
-            for (size_t q : gi) out.insert(out.end(), wg.begin() + 4 * q, wg.begin() + 4 * q + 4);
+            const int nxcd = (int)std::min<int64_t>(64, tunable("SCW_MODAL_XCD", 8));
+            if (nxcd > 1) {
+                // Workgroups are dispatched round-robin over the XCDs (workgroup
+                // b on XCD b mod nxcd), each with its own L2.  Every (run, slot)
+                // group goes whole to one XCD, so its workgroups read their
+                // shared phase fragments through one L2; groups are dealt
+                // longest first to the least-loaded XCD, and the XCD queues are
+                // interleaved back into dispatch order, padded with empty
+                // workgroups (nz = 0: the kernel returns at once).
+                struct Group { size_t first, n; int len; };
+                std::vector<Group> gs;
+                for (size_t q = 0; q < wg.size() / 4;) {
+                    size_t e = q;
+                    while (e < wg.size() / 4 && wg[4 * e].zf_off == wg[4 * q].zf_off && wg[4 * e].slot == wg[4 * q].slot)
+                        ++e;
+                    gs.push_back({q, e - q, wg[4 * q].c1 - wg[4 * q].c0});
+                    q = e;
+                }
+                std::stable_sort(gs.begin(), gs.end(), [](const Group& a, const Group& b) { return a.len > b.len; });
+                std::vector<std::vector<size_t>> qx((size_t)nxcd);
+                std::vector<int64_t> load((size_t)nxcd, 0);
+                for (const Group& g : gs) {
+                    const size_t x = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+                    for (size_t k = 0; k < g.n; ++k) qx[x].push_back(g.first + k);
+                    load[x] += (int64_t)g.n * g.len;
+                }
+                size_t depth = 0;
+                for (const auto& q : qx) depth = std::max(depth, q.size());
+                ModalItem none = wg[0];
+                none.nz = 0;
+                none.c0 = none.c1 = 0;
+                for (size_t k = 0; k < depth; ++k)
+                    for (int x = 0; x < nxcd; ++x) {
+                        if (k < qx[(size_t)x].size()) {
+                            const size_t q = qx[(size_t)x][k];
+                            out.insert(out.end(), wg.begin() + 4 * q, wg.begin() + 4 * q + 4);
+                        } else {
+                            out.insert(out.end(), 4, none);
+                        }
+                    }
+            } else {
+                std::vector<size_t> gi(wg.size() / 4);
+                for (size_t q = 0; q < gi.size(); ++q) gi[q] = q;
+                std::stable_sort(gi.begin(), gi.end(), [&](size_t a, size_t b) {
+                    return wg[4 * a].c1 - wg[4 * a].c0 > wg[4 * b].c1 - wg[4 * b].c0;
+                });
+                for (size_t q : gi) out.insert(out.end(), wg.begin() + 4 * q, wg.begin() + 4 * q + 4);
+            }
             ex->mitems.swap(out);
             ex->mclass_off[4] = (int64_t)ex->mitems.size();
         } else if (ex->mclass_off[4] > ex->mclass_off[3]) {

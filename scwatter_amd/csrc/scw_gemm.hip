@@ -790,27 +790,42 @@ __device__ __attribute__((always_inline)) inline void xcontract_columns(const ZM
             }
         }
         __syncthreads();
-        for (int rt = wv; rt < RT; rt += 4) {
-            const int m = min(8 * rt + (j >> 1), M - 1);
-            floatx4 cv[XS > 0 ? XS : 1][2];
+        // Two row tiles per round trip: a wave issues the C loads of its next
+        // two row tiles together (the second clamped to the last tile, its
+        // result then unused).  When C_z comes from HBM rather than a cache,
+        // as after another kernel has streamed through them, each round trip
+        // is long, and this halves their number (64-slab batch, BF16x3 main
+        // GEMM after a 256 MiB copy: 1.73 -> 1.51 ms; 1.10 ms warm either way).
+        constexpr int NB = 2;
+        for (int rb = wv; rb < RT; rb += 4 * NB) {
+            floatx4 cv[NB][XS > 0 ? XS : 1][2];
 #pragma unroll
-            for (int t = 0; t < XS; ++t)  // unconditional: steps past T re-read the last one
-                cs_row4(zm, R, Cz, m, 16 * min(t, T - 1) + 4 * g, cv[t][0], cv[t][1]);
-            floatx4 acc[3];
+            for (int b = 0; b < NB; ++b) {
+                const int m = min(8 * min(rb + 4 * b, RT - 1) + (j >> 1), M - 1);
 #pragma unroll
-            for (int ct = 0; ct < 3; ++ct) acc[ct] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int t = 0; t < XS; ++t) {
-                if (t >= T) break;  // uniform
-                bf16x8 ah, al;
-                xc_a_operand(cv[t][0], cv[t][1], o, ah, al);
-#pragma unroll
-                for (int ct = 0; ct < 3; ++ct) {
-                    const XFrag& xf = xs[(t * 3 + ct) * 64 + lane];
-                    xc_mfma(acc[ct], ah, al, __builtin_bit_cast(bf16x8, xf[0]), __builtin_bit_cast(bf16x8, xf[1]));
-                }
+                for (int t = 0; t < XS; ++t)  // unconditional: steps past T re-read the last one
+                    cs_row4(zm, R, Cz, m, 16 * min(t, T - 1) + 4 * g, cv[b][t][0], cv[b][t][1]);
             }
-            store(acc, rt);
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const int rt = rb + 4 * b;
+                if (rt >= RT) break;  // uniform
+                floatx4 acc[3];
+#pragma unroll
+                for (int ct = 0; ct < 3; ++ct) acc[ct] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int t = 0; t < XS; ++t) {
+                    if (t >= T) break;  // uniform
+                    bf16x8 ah, al;
+                    xc_a_operand(cv[b][t][0], cv[b][t][1], o, ah, al);
+#pragma unroll
+                    for (int ct = 0; ct < 3; ++ct) {
+                        const XFrag& xf = xs[(t * 3 + ct) * 64 + lane];
+                        xc_mfma(acc[ct], ah, al, __builtin_bit_cast(bf16x8, xf[0]), __builtin_bit_cast(bf16x8, xf[1]));
+                    }
+                }
+                store(acc, rt);
+            }
         }
     } else {
         for (int rt = wv; rt < RT; rt += 4) {
@@ -1323,6 +1338,7 @@ __device__ inline void modal_body_wg(const ModalItem& I, const ModalItem& L, con
                                      const ZMeta* __restrict__ zmeta, const uint32_t* __restrict__ zf,
                                      int64_t c_elems, float2* __restrict__ C, unsigned char* lds) {
     static_assert(MODE != 3, "16-bit modes");
+    if (L.nz == 0) return;  // an empty workgroup (uniform: every wave reads the same leader)
     constexpr int MT = 8, NH = MODE == 1 ? 2 : 1;  // fragment halves: high (and low)
     typedef typename Op16<MODE>::vec vec;
     uint32x4 (*zs)[MT][NH][64] = reinterpret_cast<uint32x4 (*)[MT][NH][64]>(lds);
@@ -1440,7 +1456,7 @@ __global__ __launch_bounds__(256, 2) void modal_wg_kernel(const ModalItem* __res
     // Whole workgroups only (the host pads this class to them), and the step
     // range comes from the workgroup's first item, so every wave passes the
     // same barriers.
-    if ((int)blockIdx.x * 4 + 3 >= nitems) return;
+    if ((int)blockIdx.x * 4 + 3 >= nitems || items[blockIdx.x * 4].nz == 0) return;  // nz = 0: empty workgroup
     const int it = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (tid_x() >> 6)));
     const ModalItem I = items[it];
     __shared__ __attribute__((aligned(16))) unsigned char lds[ModalWgLds<MODE>::kBytes];
