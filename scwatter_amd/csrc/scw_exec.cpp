@@ -1192,7 +1192,16 @@ int scw_test_exec_stall(scw_exec* ex, int on) {
     DeviceGuard g(ex->plan->device);
     if (g.st != SCW_OK) return g.st;
     drop_graph(ex);
-    return build_units(ex);
+    int st;
+    try {
+        st = build_units(ex);
+    } catch (const std::bad_alloc&) {
+        st = fail(SCW_ERR_NOMEM, "host allocation failed while building work units");
+    }
+    // a half-built unit table must never be launched: refuse runs until a
+    // later rebuild succeeds
+    if (st != SCW_OK) ex->invalid = true;
+    return st;
 }
 
 static int refuse_invalid(const scw_exec* ex) {
