@@ -399,7 +399,8 @@ def main():
     kpat = stage_kernels(dom, dt)
     roof["kernel"] = " + ".join(kpat)
     roof["stage"] = dom
-    roof["traffic"] = profiled_traffic(kpat) if args.workload == "c2" else None
+    summary = {"c2": "tile_sample_c2_kernels.md", "c5": "layer_stack_c5_kernels.md"}.get(args.workload)
+    roof["traffic"] = profiled_traffic(kpat, summary) if summary else None
     D = forms["direct_fp32"]
     roof_direct = {"bound": "valu", "achieved": round(D["info"]["flops"] / (D["ms"] * 1e-3) / 1e12, 2),
                    "peak": PEAK["fp32"], "unit": "TFLOP/s",
