@@ -144,7 +144,9 @@ int scw_batch_exec_create(int n, scw_plan* const* plans, const double* xs, size_
                           const double* const* zs, const size_t* nz, double region_tol, int dtype, scw_exec** exec);
 /* Launch shape of the GEMM form: the target number of coefficient work
  * items (wave-sized) and of main-GEMM workgroups; 0 keeps the default.
- * Rebuilds the work lists (drops a captured graph).  Used by the autotuner.
+ * Rebuilds the work lists (drops a captured graph), after waiting for the
+ * device: runs of the exec still in flight finish on the old tables.  Used by
+ * the autotuner.
  * If the rebuild fails (e.g. SCW_ERR_NOMEM), every run, stage, graph capture
  * and graph launch of the exec returns SCW_ERR_ARG until a later call
  * succeeds; the exec can always be destroyed. */

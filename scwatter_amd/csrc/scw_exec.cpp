@@ -1069,6 +1069,10 @@ static int rebuild_gemm(scw_exec* ex) {
     TraceRange tr("scw_exec rebuild");
     DeviceGuard g(ex->plan->device);
     if (g.st != SCW_OK) return g.st;
+    // a run of this exec may still be in flight on any stream (also one that
+    // does not synchronise with the copies below): let it finish before its
+    // tables are replaced
+    SCW_HIP(hipDeviceSynchronize());
     if (ex->graph_exec) {
         (void)hipGraphExecDestroy(ex->graph_exec);
         ex->graph_exec = nullptr;
@@ -1136,6 +1140,7 @@ int scw_exec_set_persistent(scw_exec* ex, int on, int64_t workgroups, double wai
     if (ex->invalid) return refuse_invalid(ex);
     DeviceGuard g(ex->plan->device);
     if (g.st != SCW_OK) return g.st;
+    SCW_HIP(hipDeviceSynchronize());  // a run in flight still reads the unit table and counters
     drop_graph(ex);
     for (DevBuf* b : {&ex->d_units, &ex->d_need, &ex->d_ctl}) b->release();
     ex->persistent = false;

@@ -52,6 +52,12 @@ static int plane_exec(scw_plan* P, int kind, int N, float a0, float b0, float fi
         // are reused where they are large enough
         ex = C.front().ex;
         C.erase(C.begin());
+        // its last run may still be in flight on the caller's stream
+        DeviceGuard dg(P->device);
+        if (dg.st != SCW_OK || hipDeviceSynchronize() != hipSuccess) {
+            scw_exec_destroy(ex);
+            return fail(SCW_ERR_DEVICE, "hipDeviceSynchronize before rebuilding a plane");
+        }
         st = exec_init(ex, P, g[0]->data(), g[0]->size(), g[1]->data(), g[1]->size(), g[2]->data(), g[2]->size(),
                        tol, form, dtype);
         if (st != SCW_OK) {
