@@ -260,6 +260,14 @@ int scw_test_exec_stall(scw_exec* exec, int on);
  * (the host sanitizer build).  Not for production use. */
 int scw_test_plan_host(const scw_planewave64* waves, size_t n_waves, const double* z_layers, int layers,
                        const int32_t* waves_begin, const int32_t* waves_end, int64_t* info);
+/* Test hook (no counterpart in the reference; device-free): the dispatch
+ * order the BF16x3/BF16/FP16 64-z mode contraction gives its workgroup
+ * groups over nxcd XCDs.  Group g has group_wgs[g] workgroups of
+ * group_steps[g] steps, numbered consecutively; order[k] receives the
+ * workgroup of dispatch slot k, or -1 for an empty slot; *count the slot
+ * count (SCW_ERR_ARG if it exceeds capacity). */
+int scw_test_xcd_order(int ngroups, const int64_t* group_wgs, const int64_t* group_steps, int nxcd, int64_t* order,
+                       int64_t capacity, int64_t* count);
 
 /* ---- .cw files (CoupledWaveStructure.h:7-134) --------------------------- */
 int scw_cw_load(const char* path, scw_cw** cw);

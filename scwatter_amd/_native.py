@@ -25,7 +25,7 @@ EXPORTS = (
     "scw_exec_create", "scw_exec_run", "scw_exec_run_stage", "scw_exec_run_host", "scw_exec_destroy", "scw_exec_info",
     "scw_exec_stage_info", "scw_batch_exec_create", "scw_exec_set_launch", "scw_exec_set_tiles",
     "scw_exec_graph_capture", "scw_exec_graph_launch", "scw_exec_set_persistent", "scw_exec_persistent_info",
-    "scw_exec_check", "scw_test_exec_stall", "scw_test_plan_host",
+    "scw_exec_check", "scw_test_exec_stall", "scw_test_plan_host", "scw_test_xcd_order",
     "scw_cw_load", "scw_cw_save", "scw_cw_create", "scw_cw_destroy", "scw_cw_set_incident", "scw_cw_add_layer",
     "scw_cw_counts", "scw_cw_layer", "scw_cw_unpack",
     "scw_post_scalar", "scw_post_minmax", "scw_post_color", "scw_test_fail_alloc_after", "scw_eval_planes",
@@ -98,6 +98,7 @@ def _load():
         "scw_exec_check": (ctypes.c_int, [_vp, _vp]),
         "scw_test_exec_stall": (ctypes.c_int, [_vp, ctypes.c_int]),
         "scw_test_plan_host": (ctypes.c_int, [_vp, ctypes.c_size_t, _dp, ctypes.c_int, _ip, _ip, _i64p]),
+        "scw_test_xcd_order": (ctypes.c_int, [ctypes.c_int, _i64p, _i64p, ctypes.c_int, _i64p, ctypes.c_int64, _i64p]),
         "scw_cw_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
         "scw_cw_save": (ctypes.c_int, [_vp, ctypes.c_char_p]),
         "scw_cw_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),

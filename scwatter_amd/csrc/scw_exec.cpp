@@ -220,6 +220,38 @@ static int build_units(scw_exec* ex) {
     return SCW_OK;
 }
 
+// Dispatch order of workgroup groups over nxcd XCDs (workgroup b of a launch
+// runs on XCD b mod nxcd, each XCD with its own L2).  Group g has
+// groups[g].first workgroups of groups[g].second steps each, numbered
+// consecutively in group order.  Every group goes whole to one XCD: groups
+// sorted longest first go to the least-loaded XCD (longest processing time
+// first), and the XCD queues are interleaved back into dispatch order,
+// position k * nxcd + x holding the k-th workgroup of XCD x's queue, or -1
+// past the end of a shorter queue.
+static std::vector<int64_t> xcd_group_order(const std::vector<std::pair<int64_t, int64_t>>& groups, int nxcd) {
+    std::vector<int64_t> first(groups.size() + 1, 0), gi(groups.size());
+    for (size_t g = 0; g < groups.size(); ++g) {
+        first[g + 1] = first[g] + groups[g].first;
+        gi[g] = (int64_t)g;
+    }
+    std::stable_sort(gi.begin(), gi.end(),
+                     [&](int64_t a, int64_t b) { return groups[(size_t)a].second > groups[(size_t)b].second; });
+    std::vector<std::vector<int64_t>> qx((size_t)nxcd);
+    std::vector<int64_t> load((size_t)nxcd, 0);
+    for (const int64_t g : gi) {
+        const size_t x = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+        for (int64_t k = 0; k < groups[(size_t)g].first; ++k) qx[x].push_back(first[(size_t)g] + k);
+        load[x] += groups[(size_t)g].first * groups[(size_t)g].second;
+    }
+    size_t depth = 0;
+    for (const auto& q : qx) depth = std::max(depth, q.size());
+    std::vector<int64_t> order;
+    order.reserve(depth * (size_t)nxcd);
+    for (size_t k = 0; k < depth; ++k)
+        for (int x = 0; x < nxcd; ++x) order.push_back(k < qx[(size_t)x].size() ? qx[(size_t)x][k] : -1);
+    return order;
+}
+
 // Builds every GEMM-form table of a run over the z entries ze (one per output
 // z-slice, in output order).  Regions are identified by (plan, region) pairs,
 // so a run may mix structures (batches).
@@ -731,44 +763,24 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
             // contraction 1.14 -> 1.04 ms (BF16x3), 0.99 -> 0.80 ms (FP16).
             const int nxcd = (int)std::min<int64_t>(64, tunable("SCW_MODAL_XCD", 8));
             if (nxcd > 1) {
-                // Workgroups are dispatched round-robin over the XCDs (workgroup
-                // b on XCD b mod nxcd), each with its own L2.  Every (run, slot)
-                // group goes whole to one XCD, so its workgroups read their
-                // shared phase fragments through one L2; groups are dealt
-                // longest first to the least-loaded XCD, and the XCD queues are
-                // interleaved back into dispatch order, padded with empty
-                // workgroups (nz = 0: the kernel returns at once).
-                struct Group { size_t first, n; int len; };
-                std::vector<Group> gs;
+                // each (run, slot) group of workgroups whole on one XCD (see
+                // xcd_group_order); padding slots become empty workgroups
+                // (nz = 0: the kernel returns at once)
+                std::vector<std::pair<int64_t, int64_t>> gs;  // (workgroups, steps) per group
                 for (size_t q = 0; q < wg.size() / 4;) {
                     size_t e = q;
                     while (e < wg.size() / 4 && wg[4 * e].zf_off == wg[4 * q].zf_off && wg[4 * e].slot == wg[4 * q].slot)
                         ++e;
-                    gs.push_back({q, e - q, wg[4 * q].c1 - wg[4 * q].c0});
+                    gs.push_back({(int64_t)(e - q), wg[4 * q].c1 - wg[4 * q].c0});
                     q = e;
                 }
-                std::stable_sort(gs.begin(), gs.end(), [](const Group& a, const Group& b) { return a.len > b.len; });
-                std::vector<std::vector<size_t>> qx((size_t)nxcd);
-                std::vector<int64_t> load((size_t)nxcd, 0);
-                for (const Group& g : gs) {
-                    const size_t x = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
-                    for (size_t k = 0; k < g.n; ++k) qx[x].push_back(g.first + k);
-                    load[x] += (int64_t)g.n * g.len;
-                }
-                size_t depth = 0;
-                for (const auto& q : qx) depth = std::max(depth, q.size());
                 ModalItem none = wg[0];
                 none.nz = 0;
                 none.c0 = none.c1 = 0;
-                for (size_t k = 0; k < depth; ++k)
-                    for (int x = 0; x < nxcd; ++x) {
-                        if (k < qx[(size_t)x].size()) {
-                            const size_t q = qx[(size_t)x][k];
-                            out.insert(out.end(), wg.begin() + 4 * q, wg.begin() + 4 * q + 4);
-                        } else {
-                            out.insert(out.end(), 4, none);
-                        }
-                    }
+                for (const int64_t q : xcd_group_order(gs, nxcd)) {
+                    if (q >= 0) out.insert(out.end(), wg.begin() + 4 * q, wg.begin() + 4 * q + 4);
+                    else out.insert(out.end(), 4, none);
+                }
             } else {
                 std::vector<size_t> gi(wg.size() / 4);
                 for (size_t q = 0; q < gi.size(); ++q) gi[q] = q;
@@ -1188,6 +1200,28 @@ int scw_exec_check(scw_exec* ex, void* stream) {
     SCW_HIP(hipMemcpy(err, &zero, sizeof(int), hipMemcpyHostToDevice));
     return fail(SCW_ERR_TIMEOUT, "single-launch run: a dependency wait exceeded %.0f ms; its output is invalid",
                 ex->wait_ms);
+}
+
+int scw_test_xcd_order(int ngroups, const int64_t* group_wgs, const int64_t* group_steps, int nxcd, int64_t* order,
+                       int64_t capacity, int64_t* count) {
+    if (ngroups < 0 || nxcd < 1 || nxcd > 64 || !count || (ngroups > 0 && (!group_wgs || !group_steps)))
+        return fail(SCW_ERR_ARG, "xcd order: bad arguments");
+    std::vector<std::pair<int64_t, int64_t>> gs((size_t)ngroups);
+    for (int g = 0; g < ngroups; ++g) {
+        if (group_wgs[g] < 0 || group_steps[g] < 0) return fail(SCW_ERR_ARG, "xcd order: negative group size");
+        gs[(size_t)g] = {group_wgs[g], group_steps[g]};
+    }
+    std::vector<int64_t> o;
+    try {
+        o = xcd_group_order(gs, nxcd);
+    } catch (const std::bad_alloc&) {
+        return fail(SCW_ERR_NOMEM, "host allocation failed");
+    }
+    *count = (int64_t)o.size();
+    if ((int64_t)o.size() > capacity || (!order && !o.empty()))
+        return fail(SCW_ERR_ARG, "xcd order: %zu slots exceed the capacity %lld", o.size(), (long long)capacity);
+    std::copy(o.begin(), o.end(), order);
+    return SCW_OK;
 }
 
 int scw_test_exec_stall(scw_exec* ex, int on) {
