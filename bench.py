@@ -231,8 +231,9 @@ def cpu_threads():
 def cpu_baselines(tables, xs, ys, z, units_1, desc):
     """The reference-semantics FP32 port (cpu_cw_evaluate_xy semantics) on
     whole rows of the plane z: single-threaded on about units_1 pts x waves,
-    and on all host threads on about 10 units_1.  Returns (all-thread record,
-    single-thread record)."""
+    and on all host threads on about 10 units_1, each as the median of three
+    runs of a third of that.  Returns (all-thread record, single-thread
+    record)."""
     import oracle
     r = oracle.region_of(float(np.float32(z)), tables.z_layers.astype(np.float32))
     w_r = int(tables.end[r] - tables.begin[r])
@@ -241,20 +242,25 @@ def cpu_baselines(tables, xs, ys, z, units_1, desc):
         # at least one row per thread (the port runs its rows in parallel),
         # each row cut to as many points as the sample leaves it
         per_row = xs.size * w_r
+        units = units / 3.0
         rows = int(min(ys.size, max(nthr, round(units / max(1, per_row)))))
         npts = int(min(xs.size, max(1, round(units / (rows * w_r)))))
         xs32 = xs[:npts].astype(np.float32)
         ys32 = ys[:rows].astype(np.float32)
         zs32 = np.array([z], np.float32)
         n_units = float(npts * rows * w_r)
-        t0 = time.perf_counter()
-        oracle.refcpu_grid(tables.W, tables.z_layers, tables.begin, tables.end, xs32, ys32, zs32, nthreads=nthr)
-        dt = time.perf_counter() - t0
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            oracle.refcpu_grid(tables.W, tables.z_layers, tables.begin, tables.end, xs32, ys32, zs32, nthreads=nthr)
+            times.append(time.perf_counter() - t0)
+        dt = float(np.median(times))
         out.append({"value": n_units / dt, "unit": UNIT, "cores": nthr, "kind": "port",
                     "sample": f"cpu_cw_evaluate_xy semantics (FP32, {nthr} thread{'s' if nthr > 1 else ''}) on "
                               f"{rows} row(s) x {npts} points of the z={z:.3f} plane of {desc} (region {r}, "
-                              f"{w_r} waves) = {n_units:.3e} units in {dt:.1f} s", **host_info()})
-        log(f"[bench] cpu baseline ({nthr} threads): {n_units:.3e} units in {dt:.1f} s -> "
+                              f"{w_r} waves) = {n_units:.3e} units per run, median of 3 runs "
+                              f"({', '.join(f'{t:.2f}' for t in times)} s)", **host_info()})
+        log(f"[bench] cpu baseline ({nthr} threads): {n_units:.3e} units, median {dt:.2f} s of 3 runs -> "
             f"{n_units / dt:.3e} pts*waves/s")
     return out[0], out[1]
 
