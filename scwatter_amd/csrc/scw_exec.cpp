@@ -103,7 +103,7 @@ static int stage_launches(const scw_exec* ex) {
     int n = ex->items.empty() ? 0 : 1;
     for (int k = 0; k < 4; ++k) n += ex->mclass_off[k + 1] > ex->mclass_off[k] ? 1 : 0;
     const bool xstage = ex->dtype == SCW_DTYPE_FP32 ? ex->my_max > 0 : ex->ncblk > 0;  // u_mfma / slot sum
-    return n + (xstage ? 1 : 0) + (int)std::max<size_t>(1, ex->mlevel_off.size() - 1);
+    return n + (xstage ? 1 : 0) + (int)std::max<size_t>(1, ex->mk_launch.size() / 3);
 }
 
 // Work units of the single-launch kernel (kinds and fields as fused16 in
@@ -591,6 +591,32 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
         }
         ex->mlevel_off.push_back((int64_t)mzm.size());
     }
+    // The FP32 stage-form main GEMM runs each level as one launch per kernel
+    // K (largest first), over the entries regrouped by the K their chunk
+    // needs: a batch mixes order counts, and the FP32 kernels of smaller K
+    // hold fewer A-operand registers, so more of their workgroups fit a CU
+    // than under the batch's largest K (64-slab batch: main GEMM 2.39 ->
+    // 2.15 ms).  The 16-bit main GEMM, which runs at the write bandwidth
+    // either way, measured slower split (BF16x3 1.20 -> 1.26 ms, FP16 1.12 ->
+    // 1.18 ms) and stays one launch per level at the largest K, as does the
+    // single-launch kernel over the z-ordered entries.  SCW_MAIN_KSPLIT=0 / 1
+    // forces either way.
+    const bool ksplit = std::getenv("SCW_MAIN_KSPLIT") ? std::atoi(std::getenv("SCW_MAIN_KSPLIT")) != 0 : !is16;
+    auto kclass = [&](int kp) {
+        if (!ksplit) return ex->kmax;  // one launch per level, the largest K
+        if (!is16) return kp <= 16 ? 16 : kp <= 32 ? 32 : kp <= 64 ? 64 : 128;
+        return kp <= 32 ? 32 : kp <= 64 ? 64 : kp <= 96 ? 96 : 128;
+    };
+    std::vector<ZMeta> mks;
+    mks.reserve(mzm.size());
+    ex->mk_launch.clear();
+    for (size_t L = 0; L + 1 < ex->mlevel_off.size(); ++L)
+        for (const int K : {128, 96, 64, 32, 16}) {
+            const int64_t first = (int64_t)mks.size();
+            for (int64_t e = ex->mlevel_off[L]; e < ex->mlevel_off[L + 1]; ++e)
+                if (kclass(mzm[e].kp) == K) mks.push_back(mzm[e]);
+            if ((int64_t)mks.size() > first) ex->mk_launch.insert(ex->mk_launch.end(), {first, (int64_t)mks.size() - first, K});
+        }
     // Slot-sum workgroups: 256 elements of one z's C_z each, per z only as
     // many as its region needs (a batch mixes regions of very different size).
     std::vector<int2> cblk;
@@ -822,6 +848,7 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
     if ((st = ex->d_regions.upload(rd.data(), rd.size() * sizeof(RegionDev))) != SCW_OK) return st;
     if ((st = ex->d_zmeta.upload(zm.data(), zm.size() * sizeof(ZMeta))) != SCW_OK) return st;
     if ((st = ex->d_mzmeta.upload(mzm.data(), mzm.size() * sizeof(ZMeta))) != SCW_OK) return st;
+    if ((st = ex->d_mksort.upload(mks.data(), mks.size() * sizeof(ZMeta))) != SCW_OK) return st;
     if ((st = ex->d_items.upload(ex->items.data(), ex->items.size() * sizeof(CoefItem))) != SCW_OK) return st;
     if ((st = ex->d_mitems.upload(ex->mitems.data(), ex->mitems.size() * sizeof(ModalItem))) != SCW_OK) return st;
     if ((st = ex->d_Z.upload(Z.data(), Z.size() * sizeof(float))) != SCW_OK) return st;
@@ -860,9 +887,10 @@ static std::vector<ZEntry> entries(const scw_exec* ex) {
 static void release_gemm(scw_exec* ex) {
     for (DevBuf* b : {&ex->d_items, &ex->d_mitems, &ex->d_regions, &ex->d_zmeta, &ex->d_Z, &ex->d_Zf, &ex->d_X, &ex->d_C,
                       &ex->d_U, &ex->d_Yfrag, &ex->d_bfpx, &ex->d_cblk, &ex->d_units, &ex->d_need, &ex->d_ctl,
-                      &ex->d_mzmeta})
+                      &ex->d_mzmeta, &ex->d_mksort})
         b->release();
     ex->mlevel_off.clear();
+    ex->mk_launch.clear();
     ex->persistent = false;
     ex->nunits = ex->pgrid = 0;
     ex->hcblk.clear();
@@ -881,7 +909,7 @@ static void exec_free(scw_exec* ex) {
     if (ex->graph_stream) (void)hipStreamDestroy(ex->graph_stream);
     for (DevBuf* b : {&ex->d_dx, &ex->d_dy, &ex->d_z, &ex->d_zslab, &ex->d_items, &ex->d_mitems, &ex->d_regions, &ex->d_zmeta,
                       &ex->d_Z, &ex->d_Zf, &ex->d_X, &ex->d_C, &ex->d_U, &ex->d_Yfrag, &ex->d_bfpx, &ex->d_cblk,
-                      &ex->d_units, &ex->d_need, &ex->d_ctl, &ex->d_mzmeta, &ex->d_out})
+                      &ex->d_units, &ex->d_need, &ex->d_ctl, &ex->d_mzmeta, &ex->d_mksort, &ex->d_out})
         b->release();
     delete ex;
 }

@@ -979,7 +979,7 @@ __global__ __launch_bounds__(256) void main_16(const ZMeta* __restrict__ zmeta, 
 
 template <int KMAX, bool VEC>
 static void launch_main(const scw_exec* ex, dim3 grid, hipStream_t st, int ytps, float* out, int zoff) {
-    const ZMeta* zm = ex->d_mzmeta.as<ZMeta>();  // main-GEMM entries [zoff, zoff + grid.y)
+    const ZMeta* zm = ex->d_mksort.as<ZMeta>();  // main-GEMM entries [zoff, zoff + grid.y), all of kernel K KMAX
     const float2* U = ex->d_U.as<float2>();
     const int nx = (int)ex->nx, ny = (int)ex->ny;
     if (ex->dtype == SCW_DTYPE_FP32) {
@@ -1791,22 +1791,23 @@ int launch_gemm(const scw_exec* ex, float* out, hipStream_t st, int stage) {
     const int64_t ncols = 6 * ex->nx;
     const int ytps = ex->ytiles_per_split;
     const bool vec = (ncols % 4) == 0;
-    // one launch per ky-chunk level: a level adds to what the previous wrote
-    for (size_t L = 0; L + 1 < ex->mlevel_off.size(); ++L) {
-        const int zoff = (int)ex->mlevel_off[L];
-        const dim3 grid((unsigned)((ncols + 255) / 256), (unsigned)(ex->mlevel_off[L + 1] - zoff), (unsigned)ex->ysplit);
+    // per ky-chunk level (a level adds to what the previous wrote) one launch
+    // per kernel K, over that K's entries
+    for (size_t i = 0; i + 2 < ex->mk_launch.size(); i += 3) {
+        const int zoff = (int)ex->mk_launch[i];
+        const dim3 grid((unsigned)((ncols + 255) / 256), (unsigned)ex->mk_launch[i + 1], (unsigned)ex->ysplit);
 #define SCW_MAIN(K)                                                        \
     if (vec) launch_main<K, true>(ex, grid, st, ytps, out, zoff);          \
     else launch_main<K, false>(ex, grid, st, ytps, out, zoff);
         if (ex->dtype == SCW_DTYPE_FP32) {
-            switch (ex->kmax) {
+            switch (ex->mk_launch[i + 2]) {
                 case 16: SCW_MAIN(16) break;
                 case 32: SCW_MAIN(32) break;
                 case 64: SCW_MAIN(64) break;
                 default: SCW_MAIN(128) break;
             }
         } else {
-            switch (ex->kmax) {
+            switch (ex->mk_launch[i + 2]) {
                 case 32: SCW_MAIN(32) break;
                 case 64: SCW_MAIN(64) break;
                 case 96: SCW_MAIN(96) break;

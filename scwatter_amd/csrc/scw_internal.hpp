@@ -221,6 +221,11 @@ struct scw_exec {
     scw::DevBuf d_items, d_mitems, d_regions, d_zmeta, d_Z, d_Zf, d_X, d_C, d_U, d_Yfrag;
     scw::DevBuf d_mzmeta;     // main-GEMM entries, by chunk level (level 0: one per z, in z order)
     std::vector<int64_t> mlevel_off;  // first entry of each chunk level, then the entry count
+    // Stage-form main launches: the same entries regrouped, within each
+    // level, by the kernel K their chunk needs; launch i covers d_mksort
+    // entries [mk_launch[3 i], + mk_launch[3 i + 1]) with kernel K mk_launch[3 i + 2]
+    scw::DevBuf d_mksort;
+    std::vector<int64_t> mk_launch;
     scw::DevBuf d_bfpx;       // FP16: shifted exponent planes of the dense regions, then the shifts
     scw::DevBuf d_cblk;       // 16-bit dtypes: slot-sum workgroups, int2 (z, first element of C_z)
     int64_t ncblk = 0;

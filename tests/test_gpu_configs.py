@@ -166,3 +166,25 @@ def test_batch_order_classes_full(c4_classes, dtype):
     bx.destroy()
     out.free()
     _check(E, ref, TOL[dtype], f"c4 order classes {dtype}")
+
+
+@pytest.mark.parametrize("dtype", ["fp32", "bf16x3"])
+def test_batch_main_launch_per_k_bit_identical(c4_classes, dtype, monkeypatch):
+    """The main GEMM as one launch at the batch's largest K (the 16-bit
+    default) or as one launch per kernel K over the regrouped z entries (the
+    FP32 default; SCW_MAIN_KSPLIT forces either): the same field, bit for bit,
+    on the four order classes (kernel K 32 to 128)."""
+    plans, (xs, ys, zs), idx, ref, params = c4_classes
+    fields = []
+    for split in ("0", "1"):
+        monkeypatch.setenv("SCW_MAIN_KSPLIT", split)
+        bx = BatchExec(plans, xs, ys, [zs] * len(plans), dtype=dtype)
+        out = hip.DeviceArray(bx.shape, np.complex64)
+        bx.run(out.ptr)
+        hip.synchronize()
+        fields.append(out.to_host())
+        bx.destroy()
+        out.free()
+    assert np.array_equal(fields[0], fields[1])
+    E = fields[1][idx[:, 0], idx[:, 1], idx[:, 2]]
+    _check(E, ref, TOL[dtype], f"c4 order classes {dtype}, main GEMM per kernel K")
