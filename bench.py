@@ -178,6 +178,35 @@ def stage_kernels(stage, dt):
     return {"coef": coef, "xcontract": [xc], "main": [main]}[stage]
 
 
+def write_ceiling(out, stream, reps=10):
+    """GB/s of the runtime's own fill (hipMemsetAsync) over the whole output
+    buffer, timed like a step: the write bandwidth the field's bytes reach
+    when nothing is computed, a measured ceiling beside the 8 TB/s peak."""
+    from scwatter_amd import hip
+    h = hip.hip()
+    e0, e1 = hip.Event(), hip.Event()
+    for _ in range(2):
+        hip._ok(h.hipMemsetAsync(out.ptr, 0, out.nbytes, stream.handle), "hipMemsetAsync")
+    e0.record(stream)
+    for _ in range(reps):
+        hip._ok(h.hipMemsetAsync(out.ptr, 0, out.nbytes, stream.handle), "hipMemsetAsync")
+    e1.record(stream)
+    e1.synchronize()
+    return out.nbytes / (e0.elapsed_ms(e1) / reps * 1e-3) / 1e9
+
+
+def roofline_coef(stage, dt):
+    """Executed-MFMA roofline of the coefficient stage: BF16x3 executes three
+    bf16 products per algorithmic one (hi x hi, hi x lo, lo x hi), the other
+    dtypes one; FP32 against the FP32 MFMA peak."""
+    mult = 3.0 if dt == "bf16x3" else 1.0
+    peak = PEAK["fp32"] if dt == "fp32" else PEAK["bf16"]
+    a = mult * stage["flops"] / (stage["ms"] * 1e-3) / 1e12
+    return {"bound": "mfma", "achieved": round(a, 2), "peak": peak, "unit": "TFLOP/s", "frac": round(a / peak, 4),
+            "stage": "coef", "note": f"executed MFMA flops ({mult:g} per algorithmic flop) over the stage's "
+                                     "in-pipeline time; the mode contraction and the VALU term contraction"}
+
+
 def roofline(flops, nbytes, ms, peak_flops):
     t = ms * 1e-3
     t_hbm = nbytes / (PEAK["hbm"] * 1e9)
@@ -336,6 +365,8 @@ def main():
 
     # ---- GEMM form, every dtype ----
     forms = {}
+    wceil = write_ceiling(out, stream)
+    log(f"[bench] write ceiling: hipMemsetAsync over the {out.nbytes / 1e9:.2f} GB field buffer at {wceil:.0f} GB/s")
     dtypes = [d for d in args.dtypes.split(",") if d]
     for dt in dtypes:
         t0 = time.perf_counter()
@@ -407,6 +438,8 @@ def main():
     roof["stage"] = dom
     summary = {"c2": "tile_sample_c2_kernels.md", "c5": "layer_stack_c5_kernels.md"}.get(args.workload)
     roof["traffic"] = profiled_traffic(kpat, summary) if summary else None
+    roof["write_ceiling"] = round(wceil, 1)
+    roof["frac_of_write_ceiling"] = round(roof["achieved"] / wceil, 4) if roof["unit"] == "GB/s" else None
     D = forms["direct_fp32"]
     roof_direct = {"bound": "valu", "achieved": round(D["info"]["flops"] / (D["ms"] * 1e-3) / 1e12, 2),
                    "peak": PEAK["fp32"], "unit": "TFLOP/s",
@@ -425,7 +458,7 @@ def main():
                       "max_abs": f["max_abs"], **({"stages_ms": {s: v["ms"] for s, v in f["stages"].items()},
                                                    "tuned": f["tuned"]} if "stages" in f else {})}
                   for k, f in forms.items()},
-        "roofline": roof, "roofline_direct": roof_direct,
+        "roofline": roof, "roofline_coef": roofline_coef(B["stages"]["coef"], dt), "roofline_direct": roof_direct,
         "cpu_baseline": cpu_all, "cpu_baseline_single_thread": cpu_one,
     }
     print(json.dumps(line), flush=True)
@@ -541,6 +574,7 @@ def bench_batch(args):
         f"({time.perf_counter()-t0:.1f} s; oracle {t_or:.1f} thread-pool s)")
     stream = hip.Stream()
     forms = {}
+    wceil = None
     for dt in [d for d in args.dtypes.split(",") if d in ("bf16x3", "fp32", "bf16", "fp16")]:
         t1 = time.perf_counter()
         bx = BatchExec(plans, xs, ys, [zs] * len(plans), dtype=dt)
@@ -555,6 +589,10 @@ def bench_batch(args):
             stages[name] = {"ms": round(sms[st - 1], 4), "flops": fl, "bytes": by}
         pidx = np.array([(i * nzi + z, y, x) for i, z, y, x in samples])
         samp = sample_device(out, bx.shape, pidx)
+        if wceil is None:
+            wceil = write_ceiling(out, stream)
+            log(f"[bench] c4 write ceiling: hipMemsetAsync over the {out.nbytes / 1e9:.2f} GB field buffer at "
+                f"{wceil:.0f} GB/s")
         forms[f"gemm_{dt}"] = {"ms": ms, "wall_ms": wall_ms, "launch": way, "ms_by_launch": by_way,
                                "rate": info["units"] / (ms * 1e-3), "info": info, "stages": stages, "sample": samp,
                                "tuned": tuned}
@@ -620,6 +658,8 @@ def bench_batch(args):
     roof["kernel"] = " + ".join(kpat)
     roof["stage"] = dom
     roof["traffic"] = profiled_traffic(kpat, "batched_sweep_c4_kernels.md")
+    roof["write_ceiling"] = round(wceil, 1)
+    roof["frac_of_write_ceiling"] = round(roof["achieved"] / wceil, 4) if roof["unit"] == "GB/s" else None
     D = forms["direct_fp32"]
     D_flops = 36.0 * D["units"]
     roof_direct = {"bound": "valu", "achieved": round(D_flops / (D["ms"] * 1e-3) / 1e12, 2), "peak": PEAK["fp32"],
@@ -638,7 +678,7 @@ def bench_batch(args):
                       "rate": f["rate"], "rel_l2": f["rel_l2"], "max_abs": f["max_abs"],
                       **({"stages_ms": {s2: v["ms"] for s2, v in f["stages"].items()}, "tuned": f["tuned"]}
                          if "stages" in f else {"subset": f["subset"]})} for k, f in forms.items()},
-        "roofline": roof, "roofline_direct": roof_direct,
+        "roofline": roof, "roofline_coef": roofline_coef(B["stages"]["coef"], dt), "roofline_direct": roof_direct,
         "cpu_baseline": cpu_all, "cpu_baseline_single_thread": cpu_one,
         "drop_in_latency": None if args.no_drop_in else drop_in_latency(),
     }

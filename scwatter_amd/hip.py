@@ -25,6 +25,7 @@ def hip():
         "hipFree": [_vp],
         "hipMemcpy": [_vp, _vp, ctypes.c_size_t, ctypes.c_int],
         "hipMemcpyAsync": [_vp, _vp, ctypes.c_size_t, ctypes.c_int, _vp],
+        "hipMemsetAsync": [_vp, ctypes.c_int, ctypes.c_size_t, _vp],
         "hipDeviceSynchronize": [],
         "hipSetDevice": [ctypes.c_int],
         "hipGetDeviceCount": [ctypes.POINTER(ctypes.c_int)],

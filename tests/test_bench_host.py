@@ -56,3 +56,13 @@ def test_stage_patterns_are_disjoint(bench):
                     assert n not in owner, (n, owner.get(n), st)
                     owner[n] = st
         assert owner.get("scw::coef_kernel") == "coef"
+
+
+def test_coefficient_roofline_counts_executed_products(bench):
+    # 1e12 algorithmic flops in 1 ms: BF16x3 executes 3e12 -> 3 PFLOP/s would pass peak; 0.5e12 -> 1.5 PF = 0.6
+    r = bench.roofline_coef({"flops": 0.5e12, "ms": 1.0}, "bf16x3")
+    assert r["bound"] == "mfma" and r["achieved"] == pytest.approx(1500.0) and r["frac"] == pytest.approx(0.6)
+    r = bench.roofline_coef({"flops": 0.5e12, "ms": 1.0}, "fp16")
+    assert r["achieved"] == pytest.approx(500.0) and r["peak"] == 2500.0
+    r = bench.roofline_coef({"flops": 0.05e12, "ms": 1.0}, "fp32")
+    assert r["peak"] == pytest.approx(157.3) and r["frac"] == pytest.approx(50.0 / 157.3, rel=1e-3)
