@@ -136,14 +136,17 @@ static int build_units(scw_exec* ex) {
     for (int i = 0; i < nitems; i += 4) cu.push_back({ex->items[i].z0, make_int4(kCoef, i, nitems, 0)});
     for (const CoefItem& it : ex->items)
         for (int z = it.z0; z < it.z0 + it.nz; ++z) ++need[z];
+    // items per unit: four (a wave each), or the 64-z class's workgroup
+    const int ipw3 = ex->dtype != SCW_DTYPE_FP32 ? ex->modal_wg_items : 4;
     for (int k = 0; k < 4; ++k) {
         const int off = (int)ex->mclass_off[k], end = (int)ex->mclass_off[k + 1];
-        if (k == 3 && (end - off) % 4 != 0 && ex->dtype != SCW_DTYPE_FP32)
+        const int per = k == 3 ? ipw3 : 4;
+        if (k == 3 && (end - off) % per != 0 && ex->dtype != SCW_DTYPE_FP32)
             return fail(SCW_ERR_ARG, "internal: 64-z mode items not in whole workgroups");
-        for (int i = off; i < end; i += 4) {
+        for (int i = off; i < end; i += per) {
             if (ex->mitems[i].nz == 0) continue;  // an empty workgroup of the XCD-grouped order
             int z0 = ex->mitems[i].z0;
-            for (int w = 1; w < 4 && i + w < end; ++w) z0 = std::min(z0, ex->mitems[i + w].z0);
+            for (int w = 1; w < per && i + w < end; ++w) z0 = std::min(z0, ex->mitems[i + w].z0);
             cu.push_back({z0, make_int4(k < 3 ? kModal1 + k : kModalWg, i, end, 0)});
         }
     }
@@ -157,7 +160,7 @@ static int build_units(scw_exec* ex) {
     for (size_t p = 0; p < cu.size(); ++p) {
         const int4 U = cu[p].second;
         const int end = U.z;
-        for (int w = 0; w < 4 && U.y + w < end; ++w) {
+        for (int w = 0; w < (U.x == kModalWg ? ipw3 : 4) && U.y + w < end; ++w) {
             int z0, zn;
             if (U.x == kCoef) {
                 z0 = ex->items[U.y + w].z0;
@@ -759,11 +762,18 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
         for (int k = 0; k <= 4; ++k) ex->mclass_off[k] = 0;
         for (const ModalItem& it : ex->mitems) ++ex->mclass_off[cls(it) + 1];
         for (int k = 1; k <= 4; ++k) ex->mclass_off[k] += ex->mclass_off[k - 1];
-        // The 64-z class of the 16-bit modes runs one workgroup per four column
-        // chunks of one run and mode slot (the waves share the phase fragments
-        // in LDS): order it by (run, slot), keeping the column order, and pad
-        // every group to a multiple of four with items past the table (their
-        // waves read clamped columns and store nothing).
+        // The 64-z class of the 16-bit modes runs one workgroup per IPW column
+        // chunks of one run and mode slot (the waves share the phase
+        // fragments in LDS): order it by (run, slot), keeping the column
+        // order, and pad every group to a multiple of IPW with items past the
+        // table (their waves read clamped columns and store nothing).  IPW = 8
+        // (one 512-thread workgroup per CU) halves the fragment loads per
+        // MFMA of IPW = 4 (two 256-thread workgroups): 64-slab batch, FP16
+        // 0.747 -> 0.710 ms, while BF16x3, whose waves wait at the wider
+        // barrier on more MFMA work, went 0.955 -> 1.04 ms and keeps IPW = 4
+        // (SCW_MODAL_WG_ITEMS = 4 or 8 overrides).
+        const int ipw = ex->modal_wg_items =
+            tunable("SCW_MODAL_WG_ITEMS", ex->dtype == SCW_DTYPE_BF16X3 ? 4 : 8) == 8 ? 8 : 4;
         if (is16 && ex->mclass_off[4] > ex->mclass_off[3]) {
             std::vector<ModalItem> out(ex->mitems.begin(), ex->mitems.begin() + ex->mclass_off[3]);
             std::vector<ModalItem> cl(ex->mitems.begin() + ex->mclass_off[3], ex->mitems.end());
@@ -775,7 +785,7 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
                 size_t k = i;
                 while (k < cl.size() && cl[k].zf_off == cl[i].zf_off && cl[k].slot == cl[i].slot) wg.push_back(cl[k++]);
                 ModalItem pad = cl[k - 1];
-                for (size_t n = k - i; n % 4; ++n) {
+                for (size_t n = k - i; n % ipw; ++n) {
                     pad.m0 += 64;
                     wg.push_back(pad);
                 }
@@ -793,27 +803,28 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
                 // xcd_group_order); padding slots become empty workgroups
                 // (nz = 0: the kernel returns at once)
                 std::vector<std::pair<int64_t, int64_t>> gs;  // (workgroups, steps) per group
-                for (size_t q = 0; q < wg.size() / 4;) {
+                for (size_t q = 0; q < wg.size() / ipw;) {
                     size_t e = q;
-                    while (e < wg.size() / 4 && wg[4 * e].zf_off == wg[4 * q].zf_off && wg[4 * e].slot == wg[4 * q].slot)
+                    while (e < wg.size() / ipw && wg[ipw * e].zf_off == wg[ipw * q].zf_off &&
+                           wg[ipw * e].slot == wg[ipw * q].slot)
                         ++e;
-                    gs.push_back({(int64_t)(e - q), wg[4 * q].c1 - wg[4 * q].c0});
+                    gs.push_back({(int64_t)(e - q), wg[ipw * q].c1 - wg[ipw * q].c0});
                     q = e;
                 }
                 ModalItem none = wg[0];
                 none.nz = 0;
                 none.c0 = none.c1 = 0;
                 for (const int64_t q : xcd_group_order(gs, nxcd)) {
-                    if (q >= 0) out.insert(out.end(), wg.begin() + 4 * q, wg.begin() + 4 * q + 4);
-                    else out.insert(out.end(), 4, none);
+                    if (q >= 0) out.insert(out.end(), wg.begin() + ipw * q, wg.begin() + ipw * q + ipw);
+                    else out.insert(out.end(), ipw, none);
                 }
             } else {
-                std::vector<size_t> gi(wg.size() / 4);
+                std::vector<size_t> gi(wg.size() / ipw);
                 for (size_t q = 0; q < gi.size(); ++q) gi[q] = q;
                 std::stable_sort(gi.begin(), gi.end(), [&](size_t a, size_t b) {
-                    return wg[4 * a].c1 - wg[4 * a].c0 > wg[4 * b].c1 - wg[4 * b].c0;
+                    return wg[ipw * a].c1 - wg[ipw * a].c0 > wg[ipw * b].c1 - wg[ipw * b].c0;
                 });
-                for (size_t q : gi) out.insert(out.end(), wg.begin() + 4 * q, wg.begin() + 4 * q + 4);
+                for (size_t q : gi) out.insert(out.end(), wg.begin() + ipw * q, wg.begin() + ipw * q + ipw);
             }
             ex->mitems.swap(out);
             ex->mclass_off[4] = (int64_t)ex->mitems.size();

@@ -1333,7 +1333,7 @@ struct ModalWgLds {  // the double-buffered phase fragments of a step
     static constexpr int kBytes = 2 * MT * NH * 64 * (int)sizeof(uint32x4);
 };
 
-template <int MODE, bool COH>
+template <int MODE, bool COH, int NW = 4>
 __device__ inline void modal_body_wg(const ModalItem& I, const ModalItem& L, const RegionDev& R,
                                      const ZMeta* __restrict__ zmeta, const uint32_t* __restrict__ zf,
                                      int64_t c_elems, float2* __restrict__ C, unsigned char* lds) {
@@ -1367,24 +1367,26 @@ __device__ inline void modal_body_wg(const ModalItem& I, const ModalItem& L, con
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt) bfp_load<true>(ra[nt], mrow, erow, col[nt], ecol[nt]);
     };
-    // This wave's quarter of a step's fragments: row tiles 2 wv, 2 wv + 1.
-    // The steps and fragments are the workgroup leader's L (the same run and
-    // slot as this wave's item I; I supplies the columns).
-    typedef uint32x4 Stage[2][NH];
+    // This wave's share of a step's fragments: row tiles RW wv .. RW wv +
+    // RW - 1 (RW = 8 / NW).  The steps and fragments are the workgroup
+    // leader's L (the same run and slot as this wave's item I; I supplies the
+    // columns).
+    constexpr int RW = MT / NW;
+    typedef uint32x4 Stage[RW][NH];
     const uint32_t* zb = zf + L.zf_off + lane * 8;
     auto loadZ = [&](Stage& st, int t) {
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            const uint32_t* p = zb + ((int64_t)(2 * wv + u) * L.T + t) * 512;
+        for (int u = 0; u < RW; ++u) {
+            const uint32_t* p = zb + ((int64_t)(RW * wv + u) * L.T + t) * 512;
 #pragma unroll
             for (int h = 0; h < NH; ++h) st[u][h] = *reinterpret_cast<const uint32x4*>(p + 4 * h);
         }
     };
     auto storeZ = [&](const Stage& st, int buf) {
 #pragma unroll
-        for (int u = 0; u < 2; ++u)
+        for (int u = 0; u < RW; ++u)
 #pragma unroll
-            for (int h = 0; h < NH; ++h) zs[buf][2 * wv + u][h][lane] = st[u][h];
+            for (int h = 0; h < NH; ++h) zs[buf][RW * wv + u][h][lane] = st[u][h];
     };
     const int t0 = L.c0 >> 4, t1 = L.c1 >> 4;
     Stage stg;
@@ -1446,21 +1448,23 @@ __global__ __launch_bounds__(256) void modal_kernel(const ModalItem* __restrict_
     modal_body<MODE, MT, false>(I, regions[I.region], zmeta, zf, c_elems, C);
 }
 
-// The 64-z class of the 16-bit modes (modal_body_wg), two waves per SIMD.
-template <int MODE>
-__global__ __launch_bounds__(256, 2) void modal_wg_kernel(const ModalItem* __restrict__ items, int nitems,
-                                                          const RegionDev* __restrict__ regions,
-                                                          const ZMeta* __restrict__ zmeta,
-                                                          const uint32_t* __restrict__ zf, int64_t c_elems,
-                                                          float2* __restrict__ C) {
+// The 64-z class of the 16-bit modes (modal_body_wg), two waves per SIMD:
+// workgroups of NW waves (NW items), two workgroups of 4 or one of 8 per CU.
+template <int MODE, int NW>
+__global__ __launch_bounds__(64 * NW, 8 / NW) void modal_wg_kernel(const ModalItem* __restrict__ items, int nitems,
+                                                                  const RegionDev* __restrict__ regions,
+                                                                  const ZMeta* __restrict__ zmeta,
+                                                                  const uint32_t* __restrict__ zf, int64_t c_elems,
+                                                                  float2* __restrict__ C) {
     // Whole workgroups only (the host pads this class to them), and the step
     // range comes from the workgroup's first item, so every wave passes the
     // same barriers.
-    if ((int)blockIdx.x * 4 + 3 >= nitems || items[blockIdx.x * 4].nz == 0) return;  // nz = 0: empty workgroup
-    const int it = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (tid_x() >> 6)));
+    const int b0 = (int)blockIdx.x * NW;
+    if (b0 + NW - 1 >= nitems || items[b0].nz == 0) return;  // nz = 0: empty workgroup
+    const int it = __builtin_amdgcn_readfirstlane(b0 + (int)(tid_x() >> 6));
     const ModalItem I = items[it];
     __shared__ __attribute__((aligned(16))) unsigned char lds[ModalWgLds<MODE>::kBytes];
-    modal_body_wg<MODE, false>(I, items[blockIdx.x * 4], regions[I.region], zmeta, zf, c_elems, C, lds);
+    modal_body_wg<MODE, false, NW>(I, items[b0], regions[I.region], zmeta, zf, c_elems, C, lds);
 }
 
 template <int MODE>
@@ -1481,8 +1485,11 @@ static void launch_modal(const scw_exec* ex, hipStream_t st) {
             default:
                 if constexpr (MODE == 3)
                     hipLaunchKernelGGL((modal_kernel<MODE, 8>), g, dim3(256), 0, st, mi + off, (int)n, rg, zm, zf, ex->c_elems, C);
+                else if (ex->modal_wg_items == 8)
+                    hipLaunchKernelGGL((modal_wg_kernel<MODE, 8>), dim3((unsigned)(n / 8)), dim3(512), 0, st, mi + off,
+                                       (int)n, rg, zm, zf, ex->c_elems, C);
                 else
-                    hipLaunchKernelGGL((modal_wg_kernel<MODE>), g, dim3(256), 0, st, mi + off, (int)n, rg, zm, zf, ex->c_elems, C);
+                    hipLaunchKernelGGL((modal_wg_kernel<MODE, 4>), g, dim3(256), 0, st, mi + off, (int)n, rg, zm, zf, ex->c_elems, C);
                 break;
         }
     }
@@ -1534,6 +1541,7 @@ struct FusedArgs {
     const int* need;   // need[nz] wave items per z, then the slot-sum units per z
     uint64_t wait_ticks;  // wall-clock ticks a wait may last
     uint64_t wait_polls;  // and its most polls
+    int modal_wg_items;   // items per 64-z mode-contraction unit (4 or 8)
 };
 
 __device__ inline int load_count(const int* p) {
@@ -1628,12 +1636,14 @@ __global__ __launch_bounds__(256, 2) void fused16(const FusedArgs a) {
                 }
                 break;
             }
-            case kUnitModalWg: {
-                const ModalItem I = a.mitems[U.y + wv];
-                modal_body_wg<MODE, true>(I, a.mitems[U.y], a.regions[I.region], a.zmeta, a.zf, a.c_elems, a.C, lds);
-                items_done(done, I.z0, I.nz);
+            case kUnitModalWg:  // a.modal_wg_items items of one run and slot, four at a time
+                for (int h = 0; h < a.modal_wg_items; h += 4) {
+                    const ModalItem I = a.mitems[U.y + h + wv];
+                    modal_body_wg<MODE, true>(I, a.mitems[U.y + h], a.regions[I.region], a.zmeta, a.zf, a.c_elems,
+                                              a.C, lds);
+                    items_done(done, I.z0, I.nz);
+                }
                 break;
-            }
             case kUnitCsum: {  // U.z consecutive slot-sum blocks of one z
                 const int z = a.cblk[U.y].x;
                 const ZMeta zm = a.zmeta[z];
@@ -1709,6 +1719,7 @@ static FusedArgs fused_args(const scw_exec* ex, float* out) {
     a.need = ex->d_need.as<int>();
     a.wait_ticks = ex->wait_ticks;
     a.wait_polls = ex->wait_polls;
+    a.modal_wg_items = ex->modal_wg_items;
     return a;
 }
 

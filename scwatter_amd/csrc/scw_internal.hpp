@@ -218,6 +218,7 @@ struct scw_exec {
     std::vector<scw::CoefItem> items;
     std::vector<scw::ModalItem> mitems;   // grouped by row-tile class (MT = 1, 2, 4, 8)
     int64_t mclass_off[5] = {0, 0, 0, 0, 0};
+    int modal_wg_items = 4;   // 16-bit 64-z class: items (waves) per workgroup, 4 or 8
     scw::DevBuf d_items, d_mitems, d_regions, d_zmeta, d_Z, d_Zf, d_X, d_C, d_U, d_Yfrag;
     scw::DevBuf d_mzmeta;     // main-GEMM entries, by chunk level (level 0: one per z, in z order)
     std::vector<int64_t> mlevel_off;  // first entry of each chunk level, then the entry count

@@ -88,6 +88,28 @@ def test_dense_slab_mode_and_term_contraction(dtype, monkeypatch):
     _same(Exec(plan, *g, form="gemm", dtype=dtype), f"dense slab, VALU term items {dtype}")
 
 
+@pytest.mark.parametrize("dtype", DTYPES16)
+def test_mode_workgroup_widths(dtype, monkeypatch):
+    """The 64-z mode contraction in workgroups of 4 items (256 threads) or 8
+    (512 threads; the single-launch kernel runs those 4 at a time): the same
+    field bit for bit either way, and the single-launch form equal to the
+    stage form for both.  Three slabs of 3^2, 5^2 and 7^2 orders make groups of
+    1, 2 and 3 items, so the padding to whole workgroups differs."""
+    specs = [(3, 0.6, 5.0), (5, 0.9, 20.0), (7, 0.45, 35.0)]
+    xs, ys = configs.axis(-6, 6, 20), configs.axis(-6, 6, 16)
+    plans, zs_list = [], []
+    for M, lam, ang in specs:
+        t = configs.tile_sample_structure(M=M, Z=1, size=(12.0, 12.0, 2.0), lam=lam, seed=M,
+                                          incidence_deg=ang).unpack()
+        plans.append(Plan(t))
+        zs_list.append(configs.axis(-0.9, 0.9, 64))
+    outs = []
+    for items in ("4", "8"):
+        monkeypatch.setenv("SCW_MODAL_WG_ITEMS", items)
+        outs.append(_same(BatchExec(plans, xs, ys, zs_list, dtype=dtype), f"{items}-item mode workgroups {dtype}"))
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+
+
 @pytest.mark.parametrize("nzs", [(5, 7, 8), (11, 12, 9), (20, 31, 17), (64, 47, 33)])
 def test_batch_mode_classes(nzs):
     """z runs of 3-8, 9-16, 17-32 and 33-64 slices: the 1-, 2-, 4- and 8-tile
