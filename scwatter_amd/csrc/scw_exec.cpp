@@ -126,7 +126,7 @@ static int build_units(scw_exec* ex) {
                                                                                        : tunable("SCW_SINGLE_MAIN_TILES", 2)));
     const int cgroup = (int)std::max<int64_t>(1, tunable("SCW_SINGLE_CSUM_BLOCKS", 8));
     const int64_t n = ((int64_t)ex->items.size() + 3) / 4 + ((int64_t)ex->mitems.size() + 12) / 4 +
-                      (int64_t)ex->hcblk.size() + gx * nz * ex->ysplit;
+                      (int64_t)ex->hcblk.size() + nz + gx * nz * ex->ysplit;
     if (n > (int64_t)INT32_MAX - (1 << 20))
         return fail(SCW_ERR_UNSUPPORTED, "single-launch form: %lld work units exceed 32-bit indices", (long long)n);
     std::vector<int> need((size_t)(2 * nz), 0);
@@ -188,6 +188,20 @@ static int build_units(scw_exec* ex) {
         ++need[nz + z];
         c = e;
     }
+    // z whose mode contraction stores in place: one slot-sum unit without
+    // blocks (z in its last field), which waits for the z's items and passes
+    // the count on to the main units
+    std::vector<char> zin((size_t)nz, 0);
+    for (const ModalItem& it : ex->mitems)
+        if (it.cs_direct)
+            for (int z = it.z0; z < it.z0 + it.nz; ++z) zin[z] = 1;
+    for (int z = 0; z < nz; ++z)
+        if (zin[z]) {
+            const int64_t pos = last[z] < 0 ? -1 : last[z] + lag;
+            all.push_back({4 * pos + 1, make_int4(kCsum, -1, 0, z)});
+            ckey[z] = std::max(ckey[z], pos);
+            ++need[nz + z];
+        }
     for (int z = 0; z < nz; ++z) {
         const int64_t pos = ckey[z] < -1 ? -1 : ckey[z] + lag;
         for (int bz = 0; bz < ex->ysplit; ++bz)
@@ -515,6 +529,16 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
             fold[v] = 1;
             rd[v].nslots = 0;
         }
+    // 16-bit dtypes: a mode-contraction run of a region with one mode slice
+    // stores its sums straight into the slot-summed layout the main GEMM
+    // reads ([3 my][ka], zero past mx), so its z need no slot sum, which
+    // would only copy them (SCW_CS_DIRECT=0 keeps the copy).
+    const char* csd_env = std::getenv("SCW_CS_DIRECT");
+    const bool cs_direct_ok = is16 && (csd_env == nullptr || std::atoi(csd_env) != 0);
+    std::vector<char> zdirect((size_t)nz, 0);
+    for (const Run& rn : runs)
+        if (cs_direct_ok && rn.modal && rd[rn.v].nslots == 1 && !fold[rn.v])
+            for (int z = rn.z0; z < rn.z1; ++z) zdirect[z] = 1;
 
     // Per-z metadata, Z rows, C and U offsets.
     std::vector<ZMeta> zm(nz);
@@ -565,7 +589,7 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
             // and a kz index per term, and the Z row), C_z out; the GEMM reads
             // C_z and X and runs the X contraction of its columns itself
             f_main += 24.0 * (double)R.mx * R.my * nx;
-            b_u += cbytes * rd[v].nslots + cbytes;
+            if (!zdirect[z]) b_u += cbytes * rd[v].nslots + cbytes;
             if (fold[v]) b_u += 28.0 * (double)rd[v].cells * rd[v].width + 8.0 * R.mc;
             b_main += cbytes + (x_counted[v] ? 0.0 : 8.0 * R.mx * nx);
             x_counted[v] = 1;
@@ -625,6 +649,7 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
     std::vector<int2> cblk;
     if (is16)
         for (int z = 0; z < nz; ++z) {
+            if (zdirect[z]) continue;  // stored in place by the mode contraction
             const int n = 3 * zm[z].my * rd[zm[z].region].ka;
             for (int e0 = 0; e0 < n; e0 += 256) cblk.push_back(make_int2(z, e0));
         }
@@ -662,6 +687,8 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
                 it.z0 = rn.z0;
                 it.nz = nzr;
                 it.T = T;
+                it.cs_direct = zdirect[rn.z0];
+                it.pad = 0;
                 it.zf_off = (int64_t)zf.size();
                 // Z'[i][k] with rows i = (z, re|im), k = (mode c, re|im):
                 //   (re, re) = Re Z, (re, im) = -Im Z, (im, re) = Im Z, (im, im) = Re Z,
@@ -868,6 +895,9 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
     if ((st = ex->d_X.upload(X.data(), X.size() * sizeof(float))) != SCW_OK) return st;
     if ((st = ex->d_C.alloc((size_t)ex->nslots_max * std::max<int64_t>(c_off, 1) * 8)) != SCW_OK) return st;
     if ((st = ex->d_U.alloc((size_t)std::max<int64_t>(u_off, 1) * 8)) != SCW_OK) return st;
+    // runs stored in place never write the zero columns past mx of their C_z
+    if (std::find(zdirect.begin(), zdirect.end(), 1) != zdirect.end())
+        SCW_HIP(hipMemset(ex->d_U.p, 0, (size_t)std::max<int64_t>(u_off, 1) * 8));
     if (!is16) {
         if (yf32.empty()) yf32.push_back(0.0f);  // every region on the grid is empty: keep a valid pointer
         if ((st = ex->d_Yfrag.upload(yf32.data(), yf32.size() * 4)) != SCW_OK) return st;

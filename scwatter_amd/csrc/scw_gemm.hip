@@ -1192,10 +1192,35 @@ int shift_bfp_exponents(const uint8_t* src, uint8_t* dst, int64_t erow, int64_t 
 // Partial sums of a mode-contraction item into its slot of C: lane holds
 // D[i = 4 g + r][m0 + 16 nt + j]: rows 4g..4g+3 are (z, re), (z, im),
 // (z + 1, re), (z + 1, im) with z = 8 mt + 2 g.
+// An item of a run with a single slot (I.cs_direct) stores into the
+// slot-summed C_z instead ([3 my][ka] rows, row 3 b + comp for cell b mx + a),
+// which no slot sum then copies.
 template <int MT, bool COH>
-__device__ inline void modal_store(const floatx4 (&acc)[4][MT], const ModalItem& I, const ZMeta* __restrict__ zmeta,
-                                   int64_t c_elems, float2* __restrict__ C, int mcols) {
+__device__ inline void modal_store(const floatx4 (&acc)[4][MT], const ModalItem& I, const RegionDev& R,
+                                   const ZMeta* __restrict__ zmeta, int64_t c_elems, float2* __restrict__ C,
+                                   float2* __restrict__ Cs, int mcols) {
     const int lane = tid_x() & 63, j = lane & 15, g = lane >> 4;
+    if (I.cs_direct) {  // uniform
+        int off[4];  // Cs_z element of each of the lane's columns
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            const int m = min(I.m0 + 16 * nt + j, mcols - 1), cell = m / 3, b = cell / R.mx;
+            off[nt] = (3 * b + (m - 3 * cell)) * R.ka + (cell - b * R.mx);
+        }
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int zi = 8 * mt + 2 * g + h;
+                if (zi >= I.nz) continue;
+                float2* o = Cs + zmeta[I.z0 + zi].u_off;
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt)
+                    if (I.m0 + 16 * nt + j < mcols)
+                        put2<COH>(o + off[nt], make_float2(acc[nt][mt][2 * h], acc[nt][mt][2 * h + 1]));
+            }
+        return;
+    }
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
@@ -1213,7 +1238,8 @@ __device__ inline void modal_store(const floatx4 (&acc)[4][MT], const ModalItem&
 
 template <int MODE, int MT, bool COH>
 __device__ inline void modal_body(const ModalItem& I, const RegionDev& R, const ZMeta* __restrict__ zmeta,
-                                  const uint32_t* __restrict__ zf, int64_t c_elems, float2* __restrict__ C) {
+                                  const uint32_t* __restrict__ zf, int64_t c_elems, float2* __restrict__ C,
+                                  float2* __restrict__ Cs) {
     const int lane = tid_x() & 63, j = lane & 15, g = lane >> 4;
     const int mcols = 3 * R.cells;
     const int W = R.width;
@@ -1312,7 +1338,7 @@ __device__ inline void modal_body(const ModalItem& I, const RegionDev& R, const 
         step(a1, z1);
         if (++t == t1) break;
     }
-    modal_store<MT, COH>(acc, I, zmeta, c_elems, C, mcols);
+    modal_store<MT, COH>(acc, I, R, zmeta, c_elems, C, Cs, mcols);
 }
 
 // Runs of 64 z (MT = 8 row tiles) with the 16-bit modes.  All four waves of
@@ -1336,7 +1362,8 @@ struct ModalWgLds {  // the double-buffered phase fragments of a step
 template <int MODE, bool COH, int NW = 4>
 __device__ inline void modal_body_wg(const ModalItem& I, const ModalItem& L, const RegionDev& R,
                                      const ZMeta* __restrict__ zmeta, const uint32_t* __restrict__ zf,
-                                     int64_t c_elems, float2* __restrict__ C, unsigned char* lds) {
+                                     int64_t c_elems, float2* __restrict__ C, float2* __restrict__ Cs,
+                                     unsigned char* lds) {
     static_assert(MODE != 3, "16-bit modes");
     if (L.nz == 0) return;  // an empty workgroup (uniform: every wave reads the same leader)
     constexpr int MT = 8, NH = MODE == 1 ? 2 : 1;  // fragment halves: high (and low)
@@ -1431,21 +1458,21 @@ __device__ inline void modal_body_wg(const ModalItem& I, const ModalItem& L, con
         step(a1, t, 1);
         if (++t == t1) break;
     }
-    modal_store<MT, COH>(acc, I, zmeta, c_elems, C, mcols);
+    modal_store<MT, COH>(acc, I, R, zmeta, c_elems, C, Cs, mcols);
 }
 
 template <int MODE, int MT>
 __global__ __launch_bounds__(256) void modal_kernel(const ModalItem* __restrict__ items, int nitems,
                                                     const RegionDev* __restrict__ regions,
                                                     const ZMeta* __restrict__ zmeta, const uint32_t* __restrict__ zf,
-                                                    int64_t c_elems, float2* __restrict__ C) {
+                                                    int64_t c_elems, float2* __restrict__ C, float2* __restrict__ Cs) {
     // XCD-contiguous order for the short-run class only: the FP32 64-z class
     // measured slower with it (64-slab batch, 3.65 -> 4.29 ms)
     const int lb = MT == 1 ? xcd_block() : (int)blockIdx.x;
     const int it = __builtin_amdgcn_readfirstlane(lb * 4 + (int)(tid_x() >> 6));
     if (it >= nitems) return;
     const ModalItem I = items[it];
-    modal_body<MODE, MT, false>(I, regions[I.region], zmeta, zf, c_elems, C);
+    modal_body<MODE, MT, false>(I, regions[I.region], zmeta, zf, c_elems, C, Cs);
 }
 
 // The 64-z class of the 16-bit modes (modal_body_wg), two waves per SIMD:
@@ -1455,7 +1482,7 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void modal_wg_kernel(const ModalIt
                                                                   const RegionDev* __restrict__ regions,
                                                                   const ZMeta* __restrict__ zmeta,
                                                                   const uint32_t* __restrict__ zf, int64_t c_elems,
-                                                                  float2* __restrict__ C) {
+                                                                  float2* __restrict__ C, float2* __restrict__ Cs) {
     // Whole workgroups only (the host pads this class to them), and the step
     // range comes from the workgroup's first item, so every wave passes the
     // same barriers.
@@ -1464,7 +1491,7 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void modal_wg_kernel(const ModalIt
     const int it = __builtin_amdgcn_readfirstlane(b0 + (int)(tid_x() >> 6));
     const ModalItem I = items[it];
     __shared__ __attribute__((aligned(16))) unsigned char lds[ModalWgLds<MODE>::kBytes];
-    modal_body_wg<MODE, false, NW>(I, items[b0], regions[I.region], zmeta, zf, c_elems, C, lds);
+    modal_body_wg<MODE, false, NW>(I, items[b0], regions[I.region], zmeta, zf, c_elems, C, Cs, lds);
 }
 
 template <int MODE>
@@ -1474,22 +1501,23 @@ static void launch_modal(const scw_exec* ex, hipStream_t st) {
     const ZMeta* zm = ex->d_zmeta.as<ZMeta>();
     const uint32_t* zf = ex->d_Zf.as<uint32_t>();
     float2* C = ex->d_C.as<float2>();
+    float2* Cs = ex->d_U.as<float2>();  // slot-summed C_z (16-bit dtypes), for runs stored in place
     for (int k = 0; k < 4; ++k) {
         const int64_t off = ex->mclass_off[k], n = ex->mclass_off[k + 1] - off;
         if (n <= 0) continue;
         const dim3 g((unsigned)((n + 3) / 4));
         switch (k) {
-            case 0: hipLaunchKernelGGL((modal_kernel<MODE, 1>), g, dim3(256), 0, st, mi + off, (int)n, rg, zm, zf, ex->c_elems, C); break;
-            case 1: hipLaunchKernelGGL((modal_kernel<MODE, 2>), g, dim3(256), 0, st, mi + off, (int)n, rg, zm, zf, ex->c_elems, C); break;
-            case 2: hipLaunchKernelGGL((modal_kernel<MODE, 4>), g, dim3(256), 0, st, mi + off, (int)n, rg, zm, zf, ex->c_elems, C); break;
+            case 0: hipLaunchKernelGGL((modal_kernel<MODE, 1>), g, dim3(256), 0, st, mi + off, (int)n, rg, zm, zf, ex->c_elems, C, Cs); break;
+            case 1: hipLaunchKernelGGL((modal_kernel<MODE, 2>), g, dim3(256), 0, st, mi + off, (int)n, rg, zm, zf, ex->c_elems, C, Cs); break;
+            case 2: hipLaunchKernelGGL((modal_kernel<MODE, 4>), g, dim3(256), 0, st, mi + off, (int)n, rg, zm, zf, ex->c_elems, C, Cs); break;
             default:
                 if constexpr (MODE == 3)
-                    hipLaunchKernelGGL((modal_kernel<MODE, 8>), g, dim3(256), 0, st, mi + off, (int)n, rg, zm, zf, ex->c_elems, C);
+                    hipLaunchKernelGGL((modal_kernel<MODE, 8>), g, dim3(256), 0, st, mi + off, (int)n, rg, zm, zf, ex->c_elems, C, Cs);
                 else if (ex->modal_wg_items == 8)
                     hipLaunchKernelGGL((modal_wg_kernel<MODE, 8>), dim3((unsigned)(n / 8)), dim3(512), 0, st, mi + off,
-                                       (int)n, rg, zm, zf, ex->c_elems, C);
+                                       (int)n, rg, zm, zf, ex->c_elems, C, Cs);
                 else
-                    hipLaunchKernelGGL((modal_wg_kernel<MODE, 4>), g, dim3(256), 0, st, mi + off, (int)n, rg, zm, zf, ex->c_elems, C);
+                    hipLaunchKernelGGL((modal_wg_kernel<MODE, 4>), g, dim3(256), 0, st, mi + off, (int)n, rg, zm, zf, ex->c_elems, C, Cs);
                 break;
         }
     }
@@ -1594,7 +1622,7 @@ __global__ __launch_bounds__(256, 2) void fused16(const FusedArgs a) {
                 const int* p = nullptr;
                 int target = 0;
                 if (U.x == kUnitCsum) {
-                    const int z = a.cblk[U.y].x;
+                    const int z = U.z > 0 ? a.cblk[U.y].x : U.w;
                     p = done + z;
                     target = a.need[z];
                 } else if ((U.x & 0xff) == kUnitMain) {
@@ -1629,9 +1657,9 @@ __global__ __launch_bounds__(256, 2) void fused16(const FusedArgs a) {
                 if (i < U.z) {
                     const ModalItem I = a.mitems[i];
                     const RegionDev& R = a.regions[I.region];
-                    if (U.x == kUnitModal1) modal_body<MODE, 1, true>(I, R, a.zmeta, a.zf, a.c_elems, a.C);
-                    else if (U.x == kUnitModal2) modal_body<MODE, 2, true>(I, R, a.zmeta, a.zf, a.c_elems, a.C);
-                    else modal_body<MODE, 4, true>(I, R, a.zmeta, a.zf, a.c_elems, a.C);
+                    if (U.x == kUnitModal1) modal_body<MODE, 1, true>(I, R, a.zmeta, a.zf, a.c_elems, a.C, a.Cs);
+                    else if (U.x == kUnitModal2) modal_body<MODE, 2, true>(I, R, a.zmeta, a.zf, a.c_elems, a.C, a.Cs);
+                    else modal_body<MODE, 4, true>(I, R, a.zmeta, a.zf, a.c_elems, a.C, a.Cs);
                     items_done(done, I.z0, I.nz);
                 }
                 break;
@@ -1640,12 +1668,12 @@ __global__ __launch_bounds__(256, 2) void fused16(const FusedArgs a) {
                 for (int h = 0; h < a.modal_wg_items; h += 4) {
                     const ModalItem I = a.mitems[U.y + h + wv];
                     modal_body_wg<MODE, true>(I, a.mitems[U.y + h], a.regions[I.region], a.zmeta, a.zf, a.c_elems,
-                                              a.C, lds);
+                                              a.C, a.Cs, lds);
                     items_done(done, I.z0, I.nz);
                 }
                 break;
-            case kUnitCsum: {  // U.z consecutive slot-sum blocks of one z
-                const int z = a.cblk[U.y].x;
+            case kUnitCsum: {  // U.z consecutive slot-sum blocks of one z (none: z = U.w, stored in place)
+                const int z = U.z > 0 ? a.cblk[U.y].x : U.w;
                 const ZMeta zm = a.zmeta[z];
                 const RegionDev& R = a.regions[zm.region];
                 for (int k = 0; k < U.z; ++k) {

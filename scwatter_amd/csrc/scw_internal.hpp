@@ -166,6 +166,8 @@ struct ModalItem {       // one wave of the MFMA mode contraction
     int32_t slot;        // partial-sum slot
     int32_t T;           // 16-mode steps in the run's phase-fragment table
     int64_t zf_off;      // 32-bit word offset of the run's phase fragments
+    int32_t cs_direct;   // 1: the run's only slot, stored straight into the slot-summed C_z layout
+    int32_t pad;
 };
 
 struct RegionDev {      // per-region data the GEMM-form kernels read (device)

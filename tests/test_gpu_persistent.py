@@ -110,6 +110,31 @@ def test_mode_workgroup_widths(dtype, monkeypatch):
     assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
 
 
+@pytest.mark.parametrize("dtype", DTYPES16)
+def test_single_slot_runs_stored_in_place(dtype, monkeypatch):
+    """Mode-contraction runs with one mode slice store their sums straight
+    into the slot-summed C_z the main GEMM reads, with no slot sum; with
+    SCW_CS_DIRECT=0 the slot-sum kernel copies them instead.  Same field either
+    way, in the stage form and the single-launch form (whose slot-sum unit of
+    such a z has no blocks and only passes its count on).  Runs of 3-8, 17-32
+    and 33-64 z cover the 1-, 4- and 8-tile classes."""
+    specs = [(3, 0.6, 5.0, 7), (5, 0.9, 20.0, 29), (7, 0.45, 35.0, 64)]
+    xs, ys = configs.axis(-6, 6, 20), configs.axis(-6, 6, 16)
+    plans, zs_list = [], []
+    for M, lam, ang, nz in specs:
+        t = configs.tile_sample_structure(M=M, Z=1, size=(12.0, 12.0, 2.0), lam=lam, seed=M,
+                                          incidence_deg=ang).unpack()
+        plans.append(Plan(t))
+        zs_list.append(configs.axis(-0.9, 0.9, nz))
+    outs = []
+    for direct in ("0", "1"):
+        monkeypatch.setenv("SCW_CS_DIRECT", direct)
+        bx = BatchExec(plans, xs, ys, zs_list, dtype=dtype)
+        bx.set_launch(coef_items=16)  # one mode slice per region
+        outs.append(_same(bx, f"single-slot runs, in place {direct}, {dtype}"))
+    assert np.array_equal(outs[0], outs[1])
+
+
 @pytest.mark.parametrize("nzs", [(5, 7, 8), (11, 12, 9), (20, 31, 17), (64, 47, 33)])
 def test_batch_mode_classes(nzs):
     """z runs of 3-8, 9-16, 17-32 and 33-64 slices: the 1-, 2-, 4- and 8-tile
