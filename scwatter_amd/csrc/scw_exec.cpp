@@ -794,11 +794,11 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
         // fragments in LDS): order it by (run, slot), keeping the column
         // order, and pad every group to a multiple of IPW with items past the
         // table (their waves read clamped columns and store nothing).  IPW = 8
-        // (one 512-thread workgroup per CU) halves the fragment loads per
-        // MFMA of IPW = 4 (two 256-thread workgroups): 64-slab batch, FP16
-        // 0.747 -> 0.710 ms, while BF16x3, whose waves wait at the wider
-        // barrier on more MFMA work, went 0.955 -> 1.04 ms and keeps IPW = 4
-        // (SCW_MODAL_WG_ITEMS = 4 or 8 overrides).
+        // (one 512-thread workgroup per CU, its two halves in ping-pong)
+        // halves the fragment loads per MFMA of IPW = 4 (two 256-thread
+        // workgroups): 64-slab batch, FP16 0.715 -> 0.656 ms; BF16x3 went
+        // 0.953 -> 0.971 ms and keeps IPW = 4 (SCW_MODAL_WG_ITEMS = 4 or 8
+        // overrides).
         const int ipw = ex->modal_wg_items =
             tunable("SCW_MODAL_WG_ITEMS", ex->dtype == SCW_DTYPE_BF16X3 ? 4 : 8) == 8 ? 8 : 4;
         if (is16 && ex->mclass_off[4] > ex->mclass_off[3]) {

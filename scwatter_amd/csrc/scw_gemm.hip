@@ -1359,7 +1359,7 @@ struct ModalWgLds {  // the double-buffered phase fragments of a step
     static constexpr int kBytes = 2 * MT * NH * 64 * (int)sizeof(uint32x4);
 };
 
-template <int MODE, bool COH, int NW = 4>
+template <int MODE, bool COH>
 __device__ inline void modal_body_wg(const ModalItem& I, const ModalItem& L, const RegionDev& R,
                                      const ZMeta* __restrict__ zmeta, const uint32_t* __restrict__ zf,
                                      int64_t c_elems, float2* __restrict__ C, float2* __restrict__ Cs,
@@ -1394,11 +1394,10 @@ __device__ inline void modal_body_wg(const ModalItem& I, const ModalItem& L, con
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt) bfp_load<true>(ra[nt], mrow, erow, col[nt], ecol[nt]);
     };
-    // This wave's share of a step's fragments: row tiles RW wv .. RW wv +
-    // RW - 1 (RW = 8 / NW).  The steps and fragments are the workgroup
-    // leader's L (the same run and slot as this wave's item I; I supplies the
-    // columns).
-    constexpr int RW = MT / NW;
+    // This wave's quarter of a step's fragments: row tiles 2 wv, 2 wv + 1.
+    // The steps and fragments are the workgroup leader's L (the same run and
+    // slot as this wave's item I; I supplies the columns).
+    constexpr int RW = 2;
     typedef uint32x4 Stage[RW][NH];
     const uint32_t* zb = zf + L.zf_off + lane * 8;
     auto loadZ = [&](Stage& st, int t) {
@@ -1475,8 +1474,102 @@ __global__ __launch_bounds__(256) void modal_kernel(const ModalItem* __restrict_
     modal_body<MODE, MT, false>(I, regions[I.region], zmeta, zf, c_elems, C, Cs);
 }
 
-// The 64-z class of the 16-bit modes (modal_body_wg), two waves per SIMD:
-// workgroups of NW waves (NW items), two workgroups of 4 or one of 8 per CU.
+// The 64-z contraction in a ping-pong schedule: a workgroup of eight waves,
+// two per SIMD, all running the same loop of two halves per step (decode the
+// step's amplitudes and issue the next loads | barrier | the step's MFMAs and
+// the next fragments into LDS | barrier), with waves 0-3 one half behind
+// waves 4-7 (an extra barrier before the loop, and one after it for waves
+// 4-7).  While one wave of a SIMD runs its MFMAs, the other decodes.  The
+// phase fragments stay double-buffered: the buffer a wave refills after its
+// MFMAs of step t was last read by the other half's MFMAs of step t - 1, one
+// half earlier.
+template <int MODE, bool COH>
+__device__ inline void modal_body_wg_pp(const ModalItem& I, const ModalItem& L, const RegionDev& R,
+                                        const ZMeta* __restrict__ zmeta, const uint32_t* __restrict__ zf,
+                                        int64_t c_elems, float2* __restrict__ C, float2* __restrict__ Cs,
+                                        unsigned char* lds) {
+    static_assert(MODE != 3, "16-bit modes");
+    if (L.nz == 0) return;  // an empty workgroup (uniform: every wave reads the same leader)
+    constexpr int MT = 8, NH = MODE == 1 ? 2 : 1;
+    typedef typename Op16<MODE>::vec vec;
+    uint32x4 (*zs)[MT][NH][64] = reinterpret_cast<uint32x4 (*)[MT][NH][64]>(lds);
+    const int lane = tid_x() & 63, j = lane & 15, g = lane >> 4;
+    const int wv = __builtin_amdgcn_readfirstlane(tid_x() >> 6);
+    const int mcols = 3 * R.cells;
+    int col[4], ecol[4];
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        col[nt] = min(I.m0 + 16 * nt + j, mcols - 1);
+        ecol[nt] = col[nt] / 3;
+    }
+    floatx4 acc[4][MT];
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) acc[nt][mt] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    typedef RawBfp RawB[4];
+    auto loadA = [&](RawB& ra, int t) {
+        const uint32x4* mrow = reinterpret_cast<const uint32x4*>(R.bfp) + (int64_t)(4 * t + g) * R.arow;
+        const uint32_t* erow = reinterpret_cast<const uint32_t*>(R.bfp_exp) + (int64_t)(4 * t + g) * R.erow;
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) bfp_load<true>(ra[nt], mrow, erow, col[nt], ecol[nt]);
+    };
+    // this wave's eighth of a step's fragments: row tile wv
+    typedef uint32x4 Stage[NH];
+    const uint32_t* zb = zf + L.zf_off + lane * 8 + (int64_t)wv * L.T * 512;
+    auto loadZ = [&](Stage& st, int t) {
+#pragma unroll
+        for (int h = 0; h < NH; ++h) st[h] = *reinterpret_cast<const uint32x4*>(zb + (int64_t)t * 512 + 4 * h);
+    };
+    auto storeZ = [&](const Stage& st, int buf) {
+#pragma unroll
+        for (int h = 0; h < NH; ++h) zs[buf][wv][h][lane] = st[h];
+    };
+    const int t0 = L.c0 >> 4, t1 = L.c1 >> 4;
+    Stage stg;
+    loadZ(stg, t0);
+    RawB a0, a1;
+    loadA(a0, t0);
+    loadA(a1, min(t0 + 1, t1 - 1));
+    storeZ(stg, 0);
+    __syncthreads();
+    if (wv < 4) __syncthreads();  // waves 0-3 trail by half a step
+    auto step = [&](RawB& ra, int t, int buf) {
+        vec bh[4], bl[4];
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) bfp_decode<MODE>(ra[nt], bh[nt], bl[nt]);
+        loadZ(stg, min(t + 1, t1 - 1));
+        loadA(ra, min(t + 2, t1 - 1));
+        __syncthreads();
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            const vec zh = __builtin_bit_cast(vec, zs[buf][mt][0][lane]);
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                if constexpr (MODE == 1) {
+                    const vec zl = __builtin_bit_cast(vec, zs[buf][mt][NH - 1][lane]);
+                    acc[nt][mt] = mfma16<MODE>(zh, bl[nt], acc[nt][mt]);
+                    acc[nt][mt] = mfma16<MODE>(zl, bh[nt], acc[nt][mt]);
+                }
+                acc[nt][mt] = mfma16<MODE>(zh, bh[nt], acc[nt][mt]);
+            }
+        }
+        storeZ(stg, buf ^ 1);
+        __syncthreads();
+    };
+    for (int t = t0;;) {
+        step(a0, t, 0);
+        if (++t == t1) break;
+        step(a1, t, 1);
+        if (++t == t1) break;
+    }
+    if (wv >= 4) __syncthreads();  // the barrier count of waves 0-3
+    modal_store<MT, COH>(acc, I, R, zmeta, c_elems, C, Cs, mcols);
+}
+
+// The 64-z class of the 16-bit modes, two waves per SIMD: workgroups of NW
+// waves (NW items), two workgroups of 4 per CU (modal_body_wg) or one of 8
+// in the ping-pong schedule (modal_body_wg_pp).
 template <int MODE, int NW>
 __global__ __launch_bounds__(64 * NW, 8 / NW) void modal_wg_kernel(const ModalItem* __restrict__ items, int nitems,
                                                                   const RegionDev* __restrict__ regions,
@@ -1491,7 +1584,10 @@ __global__ __launch_bounds__(64 * NW, 8 / NW) void modal_wg_kernel(const ModalIt
     const int it = __builtin_amdgcn_readfirstlane(b0 + (int)(tid_x() >> 6));
     const ModalItem I = items[it];
     __shared__ __attribute__((aligned(16))) unsigned char lds[ModalWgLds<MODE>::kBytes];
-    modal_body_wg<MODE, false, NW>(I, items[b0], regions[I.region], zmeta, zf, c_elems, C, Cs, lds);
+    if constexpr (NW == 8)
+        modal_body_wg_pp<MODE, false>(I, items[b0], regions[I.region], zmeta, zf, c_elems, C, Cs, lds);
+    else
+        modal_body_wg<MODE, false>(I, items[b0], regions[I.region], zmeta, zf, c_elems, C, Cs, lds);
 }
 
 template <int MODE>
