@@ -173,7 +173,7 @@ def stage_kernels(stage, dt):
     main = r"scw::main_f32<\d+, (true|false)>" if dt == "fp32" else r"scw::main_16<\d+, (true|false), %d>" % mode
     coef = [r"scw::coef_kernel", r"scw::modal_kernel<%d, \d>" % mode]
     if dt != "fp32":
-        coef.append(r"scw::modal_wg_kernel<%d>" % mode)
+        coef.append(r"scw::modal_wg_kernel<%d(, \d)?>" % mode)
     xc = r"scw::u_mfma" if dt == "fp32" else r"scw::csum_kernel"  # 16-bit: slot sum; X inside main_16
     return {"coef": coef, "xcontract": [xc], "main": [main]}[stage]
 

@@ -47,7 +47,8 @@ def test_profiled_traffic_found_for_every_bf16x3_stage(bench, stage):
 def test_stage_patterns_are_disjoint(bench):
     import re
     names = ["scw::modal_kernel<1, 1>", "scw::coef_kernel", "scw::csum_kernel", "scw::main_16<64, true, 1>",
-             "scw::modal_wg_kernel<1>", "scw::main_f32<64, true>", "scw::u_mfma"]
+             "scw::modal_wg_kernel<1>", "scw::modal_wg_kernel<1, 4>", "scw::modal_wg_kernel<2, 8>",
+             "scw::main_f32<64, true>", "scw::u_mfma"]
     for dt in ("bf16x3", "fp32"):
         owner = {}
         for st in ("coef", "xcontract", "main"):
@@ -56,6 +57,8 @@ def test_stage_patterns_are_disjoint(bench):
                     assert n not in owner, (n, owner.get(n), st)
                     owner[n] = st
         assert owner.get("scw::coef_kernel") == "coef"
+        if dt == "bf16x3":
+            assert owner.get("scw::modal_wg_kernel<1, 4>") == "coef"
 
 
 def test_coefficient_roofline_counts_executed_products(bench):
