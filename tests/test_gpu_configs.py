@@ -188,3 +188,46 @@ def test_batch_main_launch_per_k_bit_identical(c4_classes, dtype, monkeypatch):
     assert np.array_equal(fields[0], fields[1])
     E = fields[1][idx[:, 0], idx[:, 1], idx[:, 2]]
     _check(E, ref, TOL[dtype], f"c4 order classes {dtype}, main GEMM per kernel K")
+
+
+# Points per instance of the whole-sweep test (64 instances: 4096 in all).
+SWEEP_POINTS = 64
+
+
+@pytest.fixture(scope="module")
+def c4_sweep():
+    """The whole batched sweep as the bench runs it: all 64 instances of
+    config4_instances() (orders drawn from {11, 21, 31, 41}^2) on their
+    256 x 256 x 64 grids, with the FP64 oracle at 64 seeded points each."""
+    t0 = time.perf_counter()
+    params = configs.config4_instances()
+    xs, ys, zs = configs.config4_grid()
+    rng = np.random.default_rng(4096)
+    idx = [np.stack([rng.integers(0, n, SWEEP_POINTS) for n in (zs.size, ys.size, xs.size)], 1) for _ in params]
+    oracle._load()
+
+    def build(i, t):
+        pts = np.stack([xs[idx[i][:, 2]], ys[idx[i][:, 1]], zs[idx[i][:, 0]]], 1)
+        ref = oracle.eval_points_f64(t.W, t.z_layers, t.begin, t.end, pts, nthreads=max(1, NTHR // 8))
+        return Plan(t), ref
+    built = configs.map_instances(build, params, workers=8)
+    plans = [b[0] for b in built]
+    ref = np.concatenate([b[1] for b in built])
+    gidx = np.concatenate([ix + np.array([i * zs.size, 0, 0]) for i, ix in enumerate(idx)])
+    _log(f"c4 sweep: {len(plans)} instances, oracle at {len(gidx)} points ({time.perf_counter() - t0:.1f} s)")
+    return plans, (xs, ys, zs), gidx, ref
+
+
+@pytest.mark.parametrize("dtype", ["bf16x3", "fp16"])
+def test_batch_sweep_64_full(c4_sweep, dtype):
+    """All 64 instances in one batch run (6.44 GB field), sampled on the
+    device at 4,096 points against the FP64 oracle."""
+    plans, (xs, ys, zs), gidx, ref = c4_sweep
+    bx = BatchExec(plans, xs, ys, [zs] * len(plans), dtype=dtype)
+    out = hip.DeviceArray(bx.shape, np.complex64)
+    bx.run(out.ptr)
+    hip.synchronize()
+    E = hip.gather_points(out.ptr, bx.shape, gidx)
+    bx.destroy()
+    out.free()
+    _check(E, ref, TOL[dtype], f"c4 whole sweep {dtype}")
