@@ -316,7 +316,8 @@ def main():
     ap.add_argument("--retune", action="store_true", help="time the launch-shape candidates even if the "
                     "tuning table has an entry for the run")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="only run 5 device-to-device copies of 512 MiB (reference kernel for HBM counters)")
+                    help="only run the HBM counters' reference kernels on 512 MiB: 5 device-to-device copies, 5 "
+                         "fills and 5 min/max reductions")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported "
@@ -326,12 +327,24 @@ def main():
     from scwatter_amd import Exec, Plan, autotune, configs, hip
 
     if args.copy_calibration:
+        # reference kernels of known traffic for the HBM counters: a copy (n
+        # bytes read and written), the runtime's fill (n written) and the
+        # library's min/max reduction over n / 8 complex64 scalars (n read),
+        # 5 each
+        from scwatter_amd import post
         n = 512 << 20
         a, b = hip.DeviceArray((n,), np.uint8), hip.DeviceArray((n,), np.uint8)
+        mm = hip.DeviceArray((4,), np.float32)
+        stream = hip.Stream()
         for _ in range(5):
             hip._ok(hip.hip().hipMemcpy(b.ptr, a.ptr, n, hip.D2D), "hipMemcpy D2D")
+        for _ in range(5):
+            hip._ok(hip.hip().hipMemsetAsync(a.ptr, 0, n, stream.handle), "hipMemsetAsync")
+        for _ in range(5):
+            post.minmax(a.ptr, n // 8, mm.ptr, stream=stream.handle)
+        stream.synchronize()
         hip.synchronize()
-        print(json.dumps({"copy_calibration_bytes": n, "copies": 5}), flush=True)
+        print(json.dumps({"calibration_bytes": n, "copies": 5, "fills": 5, "minmax_reads": 5}), flush=True)
         return
 
     import oracle
