@@ -27,7 +27,7 @@ import numpy as np
 import pytest
 
 import oracle
-from scwatter_amd import BatchExec, Exec, Plan, configs, hip
+from scwatter_amd import BatchExec, Exec, Plan, autotune, configs, hip
 
 # the fixtures build full-size structures and their oracle samples (up to a
 # minute on the host cores), which counts towards the first test of each
@@ -218,16 +218,24 @@ def c4_sweep():
     return plans, (xs, ys, zs), gidx, ref
 
 
-@pytest.mark.parametrize("dtype", ["bf16x3", "fp16"])
-def test_batch_sweep_64_full(c4_sweep, dtype):
+@pytest.mark.parametrize("shape", ["default", "tuned"])
+@pytest.mark.parametrize("dtype", ["bf16x3", "fp32", "fp16", "bf16"])
+def test_batch_sweep_64_full(c4_sweep, dtype, shape):
     """All 64 instances in one batch run (6.44 GB field), sampled on the
-    device at 4,096 points against the FP64 oracle."""
+    device at 4,096 points against the FP64 oracle: with the library's default
+    launch shape, and with the committed tuning table's entry for this batch
+    (launch shape, mode slices, z-run tiles and form: what the bench runs)."""
     plans, (xs, ys, zs), gidx, ref = c4_sweep
     bx = BatchExec(plans, xs, ys, [zs] * len(plans), dtype=dtype)
+    if shape == "tuned":
+        entry = autotune.apply_cached(bx, autotune.shape_key(bx, "c4"))
+        assert entry is not None, f"no tuning-table entry for the batched sweep in {dtype}"
+        shown = {k: entry.get(k) for k in ("coef_items", "main_blocks", "modal_run", "coef_run", "single")}
+        _log(f"c4 whole sweep {dtype}: tuned entry {shown}")
     out = hip.DeviceArray(bx.shape, np.complex64)
     bx.run(out.ptr)
     hip.synchronize()
     E = hip.gather_points(out.ptr, bx.shape, gidx)
     bx.destroy()
     out.free()
-    _check(E, ref, TOL[dtype], f"c4 whole sweep {dtype}")
+    _check(E, ref, TOL[dtype], f"c4 whole sweep {dtype}, {shape} launch shape")
