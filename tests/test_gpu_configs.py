@@ -69,9 +69,18 @@ class Workload:
         _log(f"{name}: {tables.W.shape[0]} waves, grid {xs.size}x{ys.size}x{zs.size}, oracle at {NPTS} points "
              f"({time.perf_counter() - t0:.1f} s)")
 
-    def run(self, form, dtype):
+    def run(self, form, dtype, tuned=None):
+        """Evaluate in `form`/`dtype`; with `tuned` (a workload key of the
+        committed tuning table), under that table's entry for this exec: the
+        launch shape, z-run tiles and form (stage by stage or single launch)
+        that the bench runs."""
         ex = Exec(self.plan, self.xs, self.ys, self.zs, form=form, dtype=dtype)
         assert ex.info()["form"] == form
+        if tuned:
+            entry = autotune.apply_cached(ex, autotune.shape_key(ex, tuned))
+            assert entry is not None, f"no tuning-table entry for {tuned} in {dtype}"
+            shown = {k: entry.get(k) for k in ("coef_items", "main_blocks", "modal_run", "coef_run", "single")}
+            _log(f"{self.name} {dtype}: tuned entry {shown}")
         ex.run(self.out.ptr)
         hip.synchronize()
         E = hip.gather_points(self.out.ptr, self.shape, self.idx)
@@ -120,6 +129,16 @@ def test_pupil_41_two_polarisations_full(c3, form, dtype):
 def test_layer_stack_16_full(c5, form, dtype):
     assert c5.plan.layers == 17
     _check(c5.run(form, dtype), c5.ref, 2e-5 if form == "direct" else TOL[dtype], f"c5 {form} {dtype}")
+
+
+@pytest.mark.parametrize("workload,dtype", [(w, d) for w in ("c2", "c3") for d in ("bf16x3", "fp32", "bf16", "fp16")] +
+                         [("c5", d) for d in ("bf16x3", "fp32", "fp16")])
+def test_full_size_tuned_shapes(request, workload, dtype):
+    """The GEMM form under the committed tuning table's entry for each
+    full-size workload, as the bench runs it (the 16-layer stack in BF16x3
+    takes the single-launch form), on the same oracle sample and bars."""
+    w = request.getfixturevalue(workload)
+    _check(w.run("gemm", dtype, tuned=workload), w.ref, TOL[dtype], f"{workload} gemm {dtype}, tuned launch shape")
 
 
 # Sample points per instance of the batch test (4096 in all; fewer on the
