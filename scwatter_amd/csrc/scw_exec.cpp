@@ -220,6 +220,8 @@ static int build_units(scw_exec* ex) {
     const size_t ctl = (size_t)(kCtlWords + 2 * nz) * sizeof(int);
     if ((st = ex->d_ctl.alloc(ctl)) != SCW_OK) return st;
     SCW_HIP(hipMemset(ex->d_ctl.p, 0, ctl));
+    // complete before the first run, whatever stream the caller runs on
+    SCW_HIP(hipStreamSynchronize(nullptr));
     ex->nunits = (int64_t)units.size();
     int per_cu = 0, cus = 0, khz = 0, dev = 0;
     if ((st = fused_launch(ex, nullptr, nullptr, &per_cu)) != SCW_OK) return st;
@@ -895,9 +897,13 @@ static int prep_gemm(scw_exec* ex, const double* xs, const double* ys, const std
     if ((st = ex->d_X.upload(X.data(), X.size() * sizeof(float))) != SCW_OK) return st;
     if ((st = ex->d_C.alloc((size_t)ex->nslots_max * std::max<int64_t>(c_off, 1) * 8)) != SCW_OK) return st;
     if ((st = ex->d_U.alloc((size_t)std::max<int64_t>(u_off, 1) * 8)) != SCW_OK) return st;
-    // runs stored in place never write the zero columns past mx of their C_z
-    if (std::find(zdirect.begin(), zdirect.end(), 1) != zdirect.end())
+    // runs stored in place never write the zero columns past mx of their C_z;
+    // the fill (on the null stream) is complete before any run, whatever
+    // stream the caller runs on
+    if (std::find(zdirect.begin(), zdirect.end(), 1) != zdirect.end()) {
         SCW_HIP(hipMemset(ex->d_U.p, 0, (size_t)std::max<int64_t>(u_off, 1) * 8));
+        SCW_HIP(hipStreamSynchronize(nullptr));
+    }
     if (!is16) {
         if (yf32.empty()) yf32.push_back(0.0f);  // every region on the grid is empty: keep a valid pointer
         if ((st = ex->d_Yfrag.upload(yf32.data(), yf32.size() * 4)) != SCW_OK) return st;
