@@ -69,3 +69,24 @@ def test_coefficient_roofline_counts_executed_products(bench):
     assert r["achieved"] == pytest.approx(500.0) and r["peak"] == 2500.0
     r = bench.roofline_coef({"flops": 0.05e12, "ms": 1.0}, "fp32")
     assert r["peak"] == pytest.approx(157.3) and r["frac"] == pytest.approx(50.0 / 157.3, rel=1e-3)
+
+
+def test_cpu_baseline_records(bench):
+    """The cpu_baseline leg on a small sample of the single-interface
+    structure: both records carry the contract's keys and name the threads
+    they used, and the units their sample states are its rows x points x
+    waves."""
+    import re
+
+    from scwatter_amd import configs
+    cw, (xs, ys, zs) = configs.config1(grid=32)
+    t = cw.unpack()
+    allthr, one = bench.cpu_baselines(t, xs, ys, float(zs[-1]), 2e5, "single interface 32^3")
+    assert one["cores"] == 1 and allthr["cores"] == bench.cpu_threads()
+    for rec in (allthr, one):
+        assert {"value", "unit", "cores", "kind", "sample"} <= set(rec)
+        assert rec["kind"] == "port" and rec["unit"] == bench.UNIT and rec["value"] > 0
+        m = re.search(r"(\d+) row\(s\) x (\d+) points .* (\d+) waves\) = ([0-9.e+]+) units per run", rec["sample"])
+        assert m, rec["sample"]
+        rows, npts, waves, units = int(m[1]), int(m[2]), int(m[3]), float(m[4])
+        assert rows >= rec["cores"] and units == pytest.approx(rows * npts * waves, rel=1e-3)
